@@ -1,0 +1,391 @@
+// gqz_dev.h -- device-side scalar primitives of the multishift QZ (gfx950).
+//
+// Every helper here is computed by ONE lane (the lane that owns the control flow of a
+// workgroup); the O(n) applications of the resulting transforms are spread over the whole
+// workgroup by the callers.  Column-major storage, 0-based indices.
+//
+// Reference semantics (NLAFET/GEVP-PDHGEQZ tree):
+//   d_lartg  ~ DLARTG (used e.g. at PDHGEQZB.f:373-406)
+//   d_larfg  ~ DLARFG (DHGEQZ5.f:135)
+//   d_shift_col ~ QZFCOL (src/kkqz/qzfcol.f:1-116), homogeneous in (alpha, beta)
+//   d_invhse3   ~ INVHSE with ITOPP=2 (src/kkqz/invhse.f:61-71)
+//   d_lagv2     ~ DLAGV2 (PDHGEQZ0.f:622-674)
+//   d_tgex2     ~ DTGEX2 (src/reorder/dtgex2.f), direct swap via the generalized Sylvester eq.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GQZ_IX(i, j, ld) ((size_t)(i) + (size_t)(j) * (size_t)(ld))
+
+#define GQZ_ULP 2.220446049250313080847e-16
+#define GQZ_SAFMIN 2.2250738585072013831e-308
+
+__device__ __forceinline__ void d_lartg(double f, double g, double &c, double &s, double &r) {
+    if (g == 0.0) { c = 1.0; s = 0.0; r = f; return; }
+    if (f == 0.0) { c = 0.0; s = (g > 0.0) ? 1.0 : -1.0; r = fabs(g); return; }
+    double d = hypot(f, g);
+    double rr = copysign(d, f);
+    c = fabs(f) / d;
+    s = g / rr;
+    r = rr;
+}
+
+// Householder of length n: [alpha; x] -> [beta; 0], v = [1; x'] ; x overwritten with x'.
+__device__ inline void d_larfg(int n, double &alpha, double *x, int incx, double &tau) {
+    if (n <= 1) { tau = 0.0; return; }
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < n - 1; ++i) {
+        double v = fabs(x[(size_t)i * incx]);
+        if (v != 0.0) {
+            if (scale < v) { double r = scale / v; ssq = 1.0 + ssq * r * r; scale = v; }
+            else { double r = v / scale; ssq += r * r; }
+        }
+    }
+    double xnorm = scale * sqrt(ssq);
+    if (xnorm == 0.0) { tau = 0.0; return; }
+    double beta = -copysign(hypot(alpha, xnorm), alpha);
+    tau = (beta - alpha) / beta;
+    double sc = 1.0 / (alpha - beta);
+    for (int i = 0; i < n - 1; ++i) x[(size_t)i * incx] *= sc;
+    alpha = beta;
+}
+
+__device__ __forceinline__ void d_larfg3(double &alpha, double &x1, double &x2, double &tau) {
+    double x[2] = { x1, x2 };
+    d_larfg(3, alpha, x, 1, tau);
+    x1 = x[0]; x2 = x[1];
+}
+
+// Eigenvalues of a 2x2 pencil (A, B), B upper triangular with non-zero diagonal:
+// lambda_k = (ar_k + i ai_k), b = 1.  Standard 2x2 formula on M = B^{-1} A.
+__device__ inline void d_eig22(double a00, double a10, double a01, double a11,
+                               double b00, double b01, double b11,
+                               double ar[2], double ai[2]) {
+    double m10 = a10 / b11, m11 = a11 / b11;
+    double m00 = (a00 - b01 * m10) / b00, m01 = (a01 - b01 * m11) / b00;
+    double p = 0.5 * (m00 - m11);
+    double bcmax = fmax(fabs(m01), fabs(m10));
+    double bcmis = fmin(fabs(m01), fabs(m10)) * copysign(1.0, m01) * copysign(1.0, m10);
+    double scale = fmax(fabs(p), bcmax);
+    double z = (scale == 0.0) ? 0.0 : (p / scale) * p + (bcmax / scale) * bcmis;
+    if (z >= 0.0) {
+        double zz = p + copysign(sqrt(scale) * sqrt(z), p);
+        ar[0] = m11 + zz;
+        ar[1] = (zz != 0.0) ? m11 - (bcmax / zz) * bcmis : m11;
+        ai[0] = ai[1] = 0.0;
+    } else {
+        double re = 0.5 * (m00 + m11);
+        double disc = -((m00 - m11) * (m00 - m11) * 0.25 + m01 * m10);
+        double im = sqrt(fmax(disc, 0.0));
+        ar[0] = ar[1] = re; ai[0] = im; ai[1] = -im;
+    }
+}
+
+// First column of the double-shift polynomial of (H, T) at the top-left corner.
+// real pair: b1 b2 M^2 - (a1 b2 + a2 b1) M + a1 a2 ; complex: b1^2 M^2 - 2 b1 ar M + |a|^2.
+__device__ inline void d_shift_col(const double *H, int ldh, const double *T, int ldt,
+                                   double sr1, double si1, double sb1, double sr2, double sb2,
+                                   double v[3]) {
+    double h00 = H[GQZ_IX(0, 0, ldh)], h01 = H[GQZ_IX(0, 1, ldh)], h10 = H[GQZ_IX(1, 0, ldh)];
+    double h11 = H[GQZ_IX(1, 1, ldh)], h21 = H[GQZ_IX(2, 1, ldh)];
+    double t00 = T[GQZ_IX(0, 0, ldt)], t01 = T[GQZ_IX(0, 1, ldt)], t11 = T[GQZ_IX(1, 1, ldt)];
+    double c2, c1, c0;
+    if (si1 != 0.0) { c2 = sb1 * sb1; c1 = 2.0 * sb1 * sr1; c0 = sr1 * sr1 + si1 * si1; }
+    else { c2 = sb1 * sb2; c1 = sr1 * sb2 + sr2 * sb1; c0 = sr1 * sr2; }
+    double x0 = h00 / t00, x1 = h10 / t00;
+    double z1 = x1 / t11;
+    double z0 = (x0 - t01 * z1) / t00;
+    double y0 = h00 * z0 + h01 * z1, y1 = h10 * z0 + h11 * z1, y2 = h21 * z1;
+    v[0] = c2 * y0 - c1 * x0 + c0;
+    v[1] = c2 * y1 - c1 * x1;
+    v[2] = c2 * y2;
+    double m = fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
+    if (m > 0.0 && isfinite(m)) { v[0] /= m; v[1] /= m; v[2] /= m; }
+}
+
+// Opposite Householder of a 3x3 block of T: its first column spans the vector orthogonal to
+// rows 1 and 2 (first row of the orthogonal factor of an RQ decomposition).
+__device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &tau) {
+    double t[9];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) t[i + 3 * j] = T[GQZ_IX(i, j, ldt)];
+    double a = t[8];
+    double x[2] = { t[2], t[5] };
+    double tau1;
+    d_larfg(3, a, x, 1, tau1);
+    double u1[3] = { x[0], x[1], 1.0 };
+    for (int i = 0; i < 3; ++i) {
+        double s = tau1 * (t[i] * u1[0] + t[i + 3] * u1[1] + t[i + 6] * u1[2]);
+        t[i] -= s * u1[0]; t[i + 3] -= s * u1[1]; t[i + 6] -= s * u1[2];
+    }
+    double a2 = t[4];
+    double x2[1] = { t[1] };
+    double tau2;
+    d_larfg(2, a2, x2, 1, tau2);
+    double w0 = 1.0, w1 = 0.0, w2 = 0.0;
+    double s2 = tau2 * (w0 * x2[0] + w1);
+    w0 -= s2 * x2[0]; w1 -= s2;
+    double s1 = tau1 * (w0 * u1[0] + w1 * u1[1] + w2);
+    w0 -= s1 * u1[0]; w1 -= s1 * u1[1]; w2 -= s1;
+    double al = w0;
+    double xx[2] = { w1, w2 };
+    d_larfg(3, al, xx, 1, tau);
+    v[0] = 1.0; v[1] = xx[0]; v[2] = xx[1];
+}
+
+// small dense helpers on column-major m x m (m <= 4) arrays
+__device__ inline void d_mm(int m, const double *A, int ta, const double *B, int tb, double *C) {
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < m; ++k)
+                s += (ta ? A[k + m * i] : A[i + m * k]) * (tb ? B[j + m * k] : B[k + m * j]);
+            C[i + m * j] = s;
+        }
+}
+__device__ inline void d_g2_rows(double *M, int ld, int i, int c0, int c1, const double G[4]) {
+    for (int j = c0; j < c1; ++j) {
+        double x = M[GQZ_IX(i, j, ld)], y = M[GQZ_IX(i + 1, j, ld)];
+        M[GQZ_IX(i, j, ld)] = G[0] * x + G[1] * y;
+        M[GQZ_IX(i + 1, j, ld)] = G[2] * x + G[3] * y;
+    }
+}
+__device__ inline void d_g2_cols(double *M, int ld, int j, int r0, int r1, const double G[4]) {
+    for (int i = r0; i < r1; ++i) {
+        double a = M[GQZ_IX(i, j, ld)], b = M[GQZ_IX(i, j + 1, ld)];
+        M[GQZ_IX(i, j, ld)] = a * G[0] + b * G[1];
+        M[GQZ_IX(i, j + 1, ld)] = a * G[2] + b * G[3];
+    }
+}
+
+// Standardise a 2x2 pencil held in A (2x2, col-major, ld 2) and B (upper triangular):
+// A <- QL^T A ZR, B <- QL^T B ZR; real eigenvalues -> triangular, complex -> B diagonal > 0.
+// Returns 1 for a complex pair.
+__device__ inline int d_lagv2(double *A, double *B, double QL[4], double ZR[4]) {
+    double ar[2], ai[2];
+    d_eig22(A[0], A[1], A[2], A[3], B[0], B[2], B[3], ar, ai);
+    double anorm = fabs(A[0]) + fabs(A[1]) + fabs(A[2]) + fabs(A[3]);
+    double bnorm = fabs(B[0]) + fabs(B[2]) + fabs(B[3]);
+    double c, s, r;
+    if (ai[0] == 0.0) {
+        double lam = ar[0];
+        double h00 = A[0] - lam * B[0], h01 = A[2] - lam * B[2], h10 = A[1], h11 = A[3] - lam * B[3];
+        if (fabs(h00) + fabs(h01) >= fabs(h10) + fabs(h11)) d_lartg(h01, -h00, c, s, r);
+        else d_lartg(h11, -h10, c, s, r);
+        ZR[0] = c; ZR[1] = s; ZR[2] = -s; ZR[3] = c;
+        d_g2_cols(A, 2, 0, 0, 2, ZR);
+        d_g2_cols(B, 2, 0, 0, 2, ZR);
+        double na = fabs(A[0]) + fabs(A[1]), nb = fabs(B[0]) + fabs(B[1]);
+        if (na * bnorm >= nb * anorm) d_lartg(A[0], A[1], c, s, r);
+        else d_lartg(B[0], B[1], c, s, r);
+        QL[0] = c; QL[1] = s; QL[2] = -s; QL[3] = c;
+        d_g2_rows(A, 2, 0, 0, 2, QL);
+        d_g2_rows(B, 2, 0, 0, 2, QL);
+        A[1] = 0.0; B[1] = 0.0;
+        return 0;
+    }
+    double f = B[0], g = B[2], h = B[3];
+    double p = f * f, q = f * g, w = g * g + h * h;
+    double c2 = 1.0, s2 = 0.0;
+    if (q != 0.0) {
+        double theta = 0.5 * (w - p) / q;
+        double t = copysign(1.0, theta) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        c2 = 1.0 / sqrt(1.0 + t * t); s2 = t * c2;
+    }
+    ZR[0] = c2; ZR[1] = -s2; ZR[2] = s2; ZR[3] = c2;
+    d_g2_cols(A, 2, 0, 0, 2, ZR);
+    d_g2_cols(B, 2, 0, 0, 2, ZR);
+    double b0x = B[0], b0y = B[1], b1x = B[2], b1y = B[3];
+    double s0 = hypot(b0x, b0y), s1 = hypot(b1x, b1y);
+    double u0x, u0y, u1x, u1y;
+    if (s0 >= s1) {
+        u0x = b0x / s0; u0y = b0y / s0;
+        if (s1 > 0.0) { u1x = b1x / s1; u1y = b1y / s1; } else { u1x = -u0y; u1y = u0x; }
+        double d = u1x * u0x + u1y * u0y;
+        u1x -= d * u0x; u1y -= d * u0y;
+        double nn = hypot(u1x, u1y);
+        if (nn > 0.0) { u1x /= nn; u1y /= nn; } else { u1x = -u0y; u1y = u0x; }
+    } else {
+        u1x = b1x / s1; u1y = b1y / s1;
+        if (s0 > 0.0) { u0x = b0x / s0; u0y = b0y / s0; } else { u0x = u1y; u0y = -u1x; }
+        double d = u0x * u1x + u0y * u1y;
+        u0x -= d * u1x; u0y -= d * u1y;
+        double nn = hypot(u0x, u0y);
+        if (nn > 0.0) { u0x /= nn; u0y /= nn; } else { u0x = u1y; u0y = -u1x; }
+    }
+    QL[0] = u0x; QL[1] = u0y; QL[2] = u1x; QL[3] = u1y;
+    d_g2_rows(A, 2, 0, 0, 2, QL);
+    d_g2_rows(B, 2, 0, 0, 2, QL);
+    B[1] = 0.0; B[2] = 0.0;
+    return 1;
+}
+
+// (alpha_r, alpha_i, beta) of a standardised complex 2x2 block (B diagonal)
+__device__ inline void d_pair_eigs(const double *A, const double *B, double ar[2], double ai[2],
+                                   double be[2]) {
+    double er[2], ei[2];
+    d_eig22(A[0], A[1], A[2], A[3], B[0], 0.0, B[3], er, ei);
+    double b0 = B[0], b1 = B[3];
+    if (ei[0] != 0.0) {
+        ar[0] = er[0] * b0; ai[0] = fabs(ei[0]) * b0; be[0] = b0;
+        ar[1] = er[0] * b1; ai[1] = -fabs(ei[0]) * b1; be[1] = b1;
+    } else {
+        ar[0] = er[0] * b0; ai[0] = 0.0; be[0] = b0;
+        ar[1] = er[1] * b1; ai[1] = 0.0; be[1] = b1;
+    }
+}
+
+// ---- DTGEX2: swap adjacent blocks (n1, n2 in {1,2}) of the m x m corner (S, T) -------------
+// On success returns 0 and Qa, Za (m x m) with S_new = Qa^T S Za, T_new likewise (written back
+// into S, T with the lower-left block and T's lower part exactly zero).  Returns 1 when the
+// weak or strong stability test rejects the swap.
+__device__ inline double d_solve_cp(int m, double *K, double *rhs) {
+    int jpiv[8];
+    double xmax = 0.0;
+    for (int i = 0; i < m * m; ++i) xmax = fmax(xmax, fabs(K[i]));
+    double smlnum = GQZ_SAFMIN / GQZ_ULP;
+    double smin = fmax(GQZ_ULP * xmax, smlnum);
+    for (int i = 0; i < m; ++i) jpiv[i] = i;
+    for (int k = 0; k < m; ++k) {
+        int pi = k, pj = k; double pv = -1.0;
+        for (int j = k; j < m; ++j)
+            for (int i = k; i < m; ++i)
+                if (fabs(K[i + m * j]) > pv) { pv = fabs(K[i + m * j]); pi = i; pj = j; }
+        if (pi != k) {
+            for (int j = 0; j < m; ++j) { double t = K[k + m * j]; K[k + m * j] = K[pi + m * j]; K[pi + m * j] = t; }
+            double t = rhs[k]; rhs[k] = rhs[pi]; rhs[pi] = t;
+        }
+        if (pj != k) {
+            for (int i = 0; i < m; ++i) { double t = K[i + m * k]; K[i + m * k] = K[i + m * pj]; K[i + m * pj] = t; }
+            int t = jpiv[k]; jpiv[k] = jpiv[pj]; jpiv[pj] = t;
+        }
+        if (fabs(K[k + m * k]) < smin) K[k + m * k] = smin;
+        for (int i = k + 1; i < m; ++i) {
+            double f = K[i + m * k] / K[k + m * k];
+            K[i + m * k] = f;
+            for (int j = k + 1; j < m; ++j) K[i + m * j] -= f * K[k + m * j];
+            rhs[i] -= f * rhs[k];
+        }
+    }
+    double scale = 1.0, bmax = 0.0;
+    for (int i = 0; i < m; ++i) bmax = fmax(bmax, fabs(rhs[i]));
+    if (2.0 * smlnum * bmax > fabs(K[(m - 1) + m * (m - 1)])) {
+        double t = 0.5 / bmax;
+        for (int i = 0; i < m; ++i) rhs[i] *= t;
+        scale = t;
+    }
+    double x[8];
+    for (int i = m - 1; i >= 0; --i) {
+        double s = rhs[i];
+        for (int j = i + 1; j < m; ++j) s -= K[i + m * j] * x[j];
+        x[i] = s / K[i + m * i];
+    }
+    for (int i = 0; i < m; ++i) rhs[jpiv[i]] = x[i];
+    return scale;
+}
+
+__device__ inline void d_qr_full(int m, int k, double *X, double *Qo) {
+    double vs[4][4], taus[4];
+    for (int j = 0; j < k; ++j) {
+        double a = X[j + m * j];
+        double x[4];
+        for (int i = j + 1; i < m; ++i) x[i - j - 1] = X[i + m * j];
+        d_larfg(m - j, a, x, 1, taus[j]);
+        X[j + m * j] = a;
+        vs[j][j] = 1.0;
+        for (int i = j + 1; i < m; ++i) { vs[j][i] = x[i - j - 1]; X[i + m * j] = 0.0; }
+        for (int jj = j + 1; jj < k; ++jj) {
+            double s = 0.0;
+            for (int i = j; i < m; ++i) s += vs[j][i] * X[i + m * jj];
+            s *= taus[j];
+            for (int i = j; i < m; ++i) X[i + m * jj] -= s * vs[j][i];
+        }
+    }
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) Qo[i + m * j] = (i == j) ? 1.0 : 0.0;
+    for (int j = k - 1; j >= 0; --j)
+        for (int jj = 0; jj < m; ++jj) {
+            double s = 0.0;
+            for (int i = j; i < m; ++i) s += vs[j][i] * Qo[i + m * jj];
+            s *= taus[j];
+            for (int i = j; i < m; ++i) Qo[i + m * jj] -= s * vs[j][i];
+        }
+}
+
+__device__ inline int d_tgex2(int m, int n1, int n2, double *S, double *T, double *Qa, double *Za) {
+    double S0[16], T0[16];
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < m * m; ++i) { S0[i] = S[i]; T0[i] = T[i]; sa += S[i] * S[i]; sb += T[i] * T[i]; }
+    double smlnum = GQZ_SAFMIN / GQZ_ULP;
+    double tha = fmax(20.0 * GQZ_ULP * sqrt(sa), smlnum), thb = fmax(20.0 * GQZ_ULP * sqrt(sb), smlnum);
+    int nn = n1 * n2, m2 = 2 * nn;
+    double K[64], rhs[8];
+    for (int i = 0; i < 64; ++i) K[i] = 0.0;
+    for (int q = 0; q < n2; ++q)
+        for (int p = 0; p < n1; ++p) {
+            int row = p + n1 * q;
+            for (int k = 0; k < n1; ++k) {
+                K[row + m2 * (k + n1 * q)] += S0[p + m * k];
+                K[row + nn + m2 * (k + n1 * q)] += T0[p + m * k];
+            }
+            for (int k = 0; k < n2; ++k) {
+                K[row + m2 * (nn + p + n1 * k)] -= S0[(n1 + k) + m * (n1 + q)];
+                K[row + nn + m2 * (nn + p + n1 * k)] -= T0[(n1 + k) + m * (n1 + q)];
+            }
+            rhs[row] = S0[p + m * (n1 + q)];
+            rhs[row + nn] = T0[p + m * (n1 + q)];
+        }
+    double g = d_solve_cp(m2, K, rhs);
+    double XR[16], XL[16], W[16], S2[16], T2[16];
+    for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < m; ++i) {
+            XR[i + m * j] = (i < n1) ? -rhs[i + n1 * j] : ((i - n1) == j ? g : 0.0);
+            XL[i + m * j] = (i < n1) ? -rhs[nn + i + n1 * j] : ((i - n1) == j ? g : 0.0);
+        }
+    d_qr_full(m, n2, XR, Za);
+    d_qr_full(m, n2, XL, Qa);
+    d_mm(m, Qa, 1, S0, 0, W); d_mm(m, W, 0, Za, 0, S2);
+    d_mm(m, Qa, 1, T0, 0, W); d_mm(m, W, 0, Za, 0, T2);
+    double ea = 0.0, eb = 0.0;
+    for (int j = 0; j < n2; ++j)
+        for (int i = n2; i < m; ++i) { ea += S2[i + m * j] * S2[i + m * j]; eb += T2[i + m * j] * T2[i + m * j]; }
+    if (!(sqrt(ea) <= tha && sqrt(eb) <= thb)) return 1;
+    for (int j = 0; j < n2; ++j)
+        for (int i = n2; i < m; ++i) { S2[i + m * j] = 0.0; T2[i + m * j] = 0.0; }
+    int off[2] = { 0, n2 }, sz[2] = { n2, n1 };
+    for (int b = 0; b < 2; ++b) {
+        int o = off[b];
+        if (sz[b] != 2) continue;
+        double c, s, r;
+        d_lartg(T2[(o + 1) + m * (o + 1)], T2[(o + 1) + m * o], c, s, r);
+        for (int i = 0; i < m; ++i) {     // columns (o+1, o) rotation
+            double x, y;
+            x = S2[i + m * (o + 1)]; y = S2[i + m * o]; S2[i + m * (o + 1)] = c * x + s * y; S2[i + m * o] = c * y - s * x;
+            x = T2[i + m * (o + 1)]; y = T2[i + m * o]; T2[i + m * (o + 1)] = c * x + s * y; T2[i + m * o] = c * y - s * x;
+            x = Za[i + m * (o + 1)]; y = Za[i + m * o]; Za[i + m * (o + 1)] = c * x + s * y; Za[i + m * o] = c * y - s * x;
+        }
+        T2[(o + 1) + m * o] = 0.0;
+        double Ab[4] = { S2[o + m * o], S2[(o + 1) + m * o], S2[o + m * (o + 1)], S2[(o + 1) + m * (o + 1)] };
+        double Bb[4] = { T2[o + m * o], 0.0, T2[o + m * (o + 1)], T2[(o + 1) + m * (o + 1)] };
+        double QL[4], ZR[4];
+        d_lagv2(Ab, Bb, QL, ZR);
+        d_g2_rows(S2, m, o, 0, m, QL);
+        d_g2_rows(T2, m, o, 0, m, QL);
+        d_g2_cols(S2, m, o, 0, m, ZR);
+        d_g2_cols(T2, m, o, 0, m, ZR);
+        d_g2_cols(Qa, m, o, 0, m, QL);
+        d_g2_cols(Za, m, o, 0, m, ZR);
+        S2[o + m * o] = Ab[0]; S2[(o + 1) + m * o] = Ab[1]; S2[o + m * (o + 1)] = Ab[2]; S2[(o + 1) + m * (o + 1)] = Ab[3];
+        T2[o + m * o] = Bb[0]; T2[(o + 1) + m * o] = 0.0; T2[o + m * (o + 1)] = Bb[2]; T2[(o + 1) + m * (o + 1)] = Bb[3];
+    }
+    double S3[16], T3[16];
+    d_mm(m, Qa, 0, S2, 0, W); d_mm(m, W, 0, Za, 1, S3);
+    d_mm(m, Qa, 0, T2, 0, W); d_mm(m, W, 0, Za, 1, T3);
+    double da = 0.0, db = 0.0;
+    for (int i = 0; i < m * m; ++i) { da += (S3[i] - S0[i]) * (S3[i] - S0[i]); db += (T3[i] - T0[i]) * (T3[i] - T0[i]); }
+    if (!(sqrt(da) <= tha && sqrt(db) <= thb)) return 1;
+    for (int j = 0; j < m; ++j)
+        for (int i = j + 1; i < m; ++i) T2[i + m * j] = 0.0;
+    for (int i = 0; i < m * m; ++i) { S[i] = S2[i]; T[i] = T2[i]; }
+    return 0;
+}

@@ -1,0 +1,70 @@
+// gqz_kernels.h -- kernel entry points and host-side launch helpers shared by the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct GqzSmallJob {
+    double *H, *T;          // block top-left, leading dimension ld
+    int ld, m;
+    double *U, *V;          // m x m outputs (may be null), leading dimension ldu
+    int ldu, pad;
+    double *ar, *ai, *be;   // eigenvalue outputs (m entries each)
+    int *info;
+    double ttol, smlnum;
+};
+
+// dynamic LDS bytes of k_small_qz for a block of order m (H, T, U, V with ld = m + 1 + control)
+size_t gqz_small_lds_bytes(int m);
+
+#ifdef __HIPCC__
+extern "C" __global__ void k_small_qz(const GqzSmallJob *jobs);
+extern "C" __global__ void k_small_qz1(GqzSmallJob job);
+extern "C" __global__ void k_aed_post(double *Hw, double *Tw, double *U, double *V, int jw,
+                                      const double *sptr, double smlnum, double *war, double *wai,
+                                      double *wb, double *scr, int *out);
+extern "C" __global__ void k_gghrd_wg(double *A, double *B, int ld, int n, int ilo, int ihi,
+                                      double *Q, double *Z, int ldq, int nq);
+#endif
+
+// up to four in-place slab updates sharing one launch (same k): C_q <- C_q F_q (right) or
+// C_q <- F_q^T C_q (left); m[q] = rows (right) or columns (left) of slab q.
+struct GqzSlabJobs {
+    int njobs;
+    double *C[4];
+    int ldc[4];
+    int m[4];
+    const double *F[4];
+    int ldf[4];
+    int nblk[4];
+};
+void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs J, int k);
+void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs J, int k);
+
+// ---- host launchers (defined in the .hip files; all asynchronous on `st`) -----------------
+// C[m x k] <- C * V[k x k]   (in place, k <= 1024)
+void gqz_slab_right(hipStream_t st, double *C, int ldc, int m, int k, const double *V, int ldv,
+                    double *ws);
+// C[k x n] <- U^T[k x k] * C  (in place)
+void gqz_slab_left(hipStream_t st, double *C, int ldc, int k, int n, const double *U, int ldu,
+                   double *ws);
+// general out-of-place C = op(A) * op(B) (m x n, inner k); ta/tb transpose flags
+void gqz_gemm(hipStream_t st, int ta, int tb, int m, int n, int k, const double *A, int lda,
+              const double *B, int ldb, double *C, int ldc);
+
+// chain chase inside an LDS window (see k_chase.hip)
+struct GqzChainArgs {
+    double *H, *T;     // full matrices
+    int ld;
+    int p;             // window top row/col
+    int w;             // window order
+    int nb;            // bulges in the chain
+    int t0;            // tail column (window-relative) of the top bulge at entry, -1 = intro
+    int steps;         // chase steps of every bulge (capped by push-off)
+    int intro;         // 1: introduce all bulges at the top of the window first
+    int last;          // 1: window bottom is the bottom of the active block (push-off)
+    double *U, *V;     // w x w outputs (ld = w)
+    const double *sr, *si, *sb;   // 2*nb shifts
+    double smlnum;
+};
+void gqz_chase_window(hipStream_t st, const GqzChainArgs &a);
+int gqz_chase_max_window();

@@ -1,0 +1,26 @@
+// gqz_misc.h -- host launchers of the auxiliary kernels (k_misc.hip, k_wgqz.hip).
+#pragma once
+#include "gqz_kernels.h"
+
+void gqz_launch_inf_defl(hipStream_t st, double *H, double *T, double *Q, double *Z, int ld, int ldq,
+                         int n, int nq, int ilo, int kbot, double ttol, double smlnum, double *ar,
+                         double *ai, double *be, int *out);
+void gqz_launch_window_in(hipStream_t st, const double *H, const double *T, int ld, int k0, int jw,
+                          double *Hw, double *Tw, double *U, double *V);
+void gqz_launch_window_out(hipStream_t st, double *H, double *T, int ld, int k0, int jw, const double *Hw,
+                           const double *Tw, const double *U, int has_spike);
+void gqz_launch_identity(hipStream_t st, double *A, int ld, int m, int n);
+void gqz_launch_fill(hipStream_t st, double *x, int n, double v);
+void gqz_launch_finalize(hipStream_t st, double *H, double *T, double *Z, int ld, int ldz, int nz, int ilo,
+                         int ihi, double *ar, double *ai, double *be);
+void gqz_launch_small(hipStream_t st, const GqzSmallJob *djobs, int njobs, int mmax);
+void gqz_launch_aed_post(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
+                         const double *sptr, double smlnum, double *war, double *wai, double *wb,
+                         double *scr, int *out);
+void gqz_launch_small1(hipStream_t st, const GqzSmallJob &job);
+void gqz_launch_gghrd_wg(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
+                         double *Z, int ldq, int nq);
+void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
+                         double *tau);
+void gqz_launch_sumsq(hipStream_t st, const double *A, int ld, int m, int n, int upper, double *out);
+void gqz_launch_sub(hipStream_t st, double *R, int ldr, const double *S, int lds, int n);

@@ -1,0 +1,244 @@
+// k_chase.hip -- multishift sweep: a chain of tightly packed 3x3 bulges chased inside an
+// LDS-resident window by one workgroup (PDHGEQZ5/A/B + DHGEQZ5 semantics).
+//
+// Window: rows/cols [p, p+w) of H and T, w <= 64, held in LDS together with its accumulated
+// orthogonal factors U, V (w x w).  Bulge b (b = 0 top) sits at "tail column" c_b (window
+// relative): its next left reflector acts on rows c+1..c+3 and is computed from H(c+1:c+3, c)
+// (c = -1 means introduction from the shift polynomial).  Bulges are 3 columns apart.
+//
+// Every chase step advances ALL active bulges by one column in two phases:
+//   phase L: every bulge computes its left reflector (one lane each) and the workgroup applies
+//            all of them (their row sets are disjoint);
+//   phase R: every bulge computes its opposite (right) reflector from the updated 3x3 block of
+//            T and the workgroup applies all of them (disjoint column sets).
+// Left and right multiplications commute, and no reflector reads an entry that a neighbouring
+// bulge's same-step reflector writes before it in the reference order (bottom bulge first,
+// PDHGEQZB.f:357-407; introduction order of PDHGEQZA.f:290-330), so the result equals the
+// reference's sequential chase up to rounding.
+//
+// The parts of the pencil outside the window are updated afterwards by the MFMA slab kernels
+// with U (rows to the right) and V (columns above, Q, Z).
+#include "gqz_dev.h"
+#include "gqz_kernels.h"
+
+#define CH_MAXW 64
+#define CH_MAXB 21
+
+__global__ void __launch_bounds__(256) k_chase(GqzChainArgs a) {
+    extern __shared__ double lds[];
+    const int w = a.w, ld = w + 1, nb = a.nb;
+    double *sH = lds, *sT = sH + (size_t)ld * w, *sU = sT + (size_t)ld * w, *sV = sU + (size_t)ld * w;
+    double *bv = sV + (size_t)ld * w;          // per bulge: v1 v2 tau w1 w2 tau2 c s  (8)
+    int *bc = (int *)(bv + 8 * CH_MAXB);       // per bulge: current tail column
+    int *bk = bc + CH_MAXB;                    // per bulge: kind of this step (0 idle, 1 refl, 2 pushoff, 3 intro)
+    const int tid = threadIdx.x;
+    const double *gH = a.H + GQZ_IX(a.p, a.p, a.ld);
+    const double *gT = a.T + GQZ_IX(a.p, a.p, a.ld);
+    for (int idx = tid; idx < w * w; idx += blockDim.x) {
+        int i = idx % w, j = idx / w;
+        sH[i + ld * j] = gH[GQZ_IX(i, j, a.ld)];
+        sT[i + ld * j] = gT[GQZ_IX(i, j, a.ld)];
+        sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
+        sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int total = a.intro ? 3 * nb - 2 + a.steps : a.steps;
+    for (int s = 0; s < total; ++s) {
+        // ---------------- phase L: left reflectors ----------------
+        if (tid < nb) {
+            const int b = tid;
+            const int start = a.intro ? 3 * (nb - 1 - b) : 0;
+            const int nst = a.intro ? 3 * b + 1 + a.steps : a.steps;
+            const int cst = a.intro ? -1 : a.t0 + 3 * b;
+            const int q = s - start;
+            int kind = 0, c = cst + q;
+            if (q >= 0 && q < nst) {
+                if (c + 3 <= w - 1) kind = (c < 0) ? 3 : 1;
+                else if (a.last && c + 2 == w - 1) kind = 2;
+                else kind = 0;
+            }
+            // shifts of this bulge (pair index nb-1-b)
+            const int sp = 2 * (nb - 1 - b);
+            double *B = bv + 8 * b;
+            if (kind == 3) {
+                double v[3];
+                d_shift_col(sH, ld, sT, ld, a.sr[sp], a.si[sp], a.sb[sp], a.sr[sp + 1], a.sb[sp + 1], v);
+                double al = v[0], x1 = v[1], x2 = v[2], tau;
+                d_larfg3(al, x1, x2, tau);
+                B[0] = x1; B[1] = x2; B[2] = tau;
+            } else if (kind == 1) {
+                const int k = c + 1;
+                // zero the fill left in T by this bulge's previous right reflector
+                if (q > 0) { sT[k + ld * (k - 1)] = 0.0; sT[(k + 1) + ld * (k - 1)] = 0.0; }
+                double h0 = sH[k + ld * c], h1 = sH[(k + 1) + ld * c], h2 = sH[(k + 2) + ld * c];
+                double beta = h0, x1 = h1, x2 = h2, tau;
+                d_larfg3(beta, x1, x2, tau);
+                if (h2 != 0.0 || sH[(k + 2) + ld * k] != 0.0 || sH[(k + 2) + ld * (k + 1)] == 0.0) {
+                    sH[k + ld * c] = beta;
+                } else {
+                    double wv[3];
+                    d_shift_col(sH + k + ld * k, ld, sT + k + ld * k, ld, a.sr[sp], a.si[sp], a.sb[sp],
+                                a.sr[sp + 1], a.sb[sp + 1], wv);
+                    double a1 = wv[0], y1 = wv[1], y2 = wv[2], tau1;
+                    d_larfg3(a1, y1, y2, tau1);
+                    double refsum = tau1 * (h0 + y1 * h1);
+                    double tst = fabs(sH[c + ld * c]) + fabs(sH[k + ld * k]) + fabs(sH[(k + 1) + ld * (k + 1)]);
+                    if (fabs(h1 - refsum * y1) + fabs(refsum * y2) > GQZ_ULP * tst) {
+                        sH[k + ld * c] = beta;
+                    } else {
+                        sH[k + ld * c] = h0 - refsum;
+                        tau = tau1; x1 = y1; x2 = y2;
+                    }
+                }
+                sH[(k + 1) + ld * c] = 0.0; sH[(k + 2) + ld * c] = 0.0;
+                B[0] = x1; B[1] = x2; B[2] = tau;
+            } else if (kind == 2) {
+                const int l = w - 1;
+                if (q > 0) { sT[(l - 1) + ld * (l - 2)] = 0.0; sT[l + ld * (l - 2)] = 0.0; }
+                double cs, sn, r;
+                d_lartg(sH[(l - 1) + ld * (l - 2)], sH[l + ld * (l - 2)], cs, sn, r);
+                sH[(l - 1) + ld * (l - 2)] = r; sH[l + ld * (l - 2)] = 0.0;
+                B[6] = cs; B[7] = sn;
+            }
+            bc[b] = c; bk[b] = kind;
+        }
+        __syncthreads();
+        // apply all left transforms: per bulge 3w items (H cols, T cols, U rows)
+        for (int idx = tid; idx < nb * 3 * w; idx += blockDim.x) {
+            const int b = idx / (3 * w), r = idx % (3 * w);
+            const int kind = bk[b];
+            if (kind == 0) continue;
+            const int c = bc[b];
+            const double *B = bv + 8 * b;
+            if (kind == 2) {
+                const int i = w - 2;
+                const double cs = B[6], sn = B[7];
+                if (r < w) {                         // H cols [w-2, w)
+                    const int j = r;
+                    if (j < w - 2) continue;
+                    double x = sH[i + ld * j], y = sH[(i + 1) + ld * j];
+                    sH[i + ld * j] = cs * x + sn * y; sH[(i + 1) + ld * j] = cs * y - sn * x;
+                } else if (r < 2 * w) {
+                    const int j = r - w;
+                    if (j < w - 2) continue;
+                    double x = sT[i + ld * j], y = sT[(i + 1) + ld * j];
+                    sT[i + ld * j] = cs * x + sn * y; sT[(i + 1) + ld * j] = cs * y - sn * x;
+                } else {
+                    const int row = r - 2 * w;
+                    double x = sU[row + ld * i], y = sU[row + ld * (i + 1)];
+                    sU[row + ld * i] = cs * x + sn * y; sU[row + ld * (i + 1)] = cs * y - sn * x;
+                }
+                continue;
+            }
+            const int k = c + 1;
+            const double v1 = B[0], v2 = B[1], tau = B[2];
+            if (tau == 0.0) continue;
+            if (r < 2 * w) {
+                double *M = r < w ? sH : sT;
+                const int j = r < w ? r : r - w;
+                if (j < k) continue;                // columns k.. (column c set explicitly)
+                double *p = M + k + ld * j;
+                double t = tau * (p[0] + v1 * p[1] + v2 * p[2]);
+                p[0] -= t; p[1] -= t * v1; p[2] -= t * v2;
+            } else {
+                const int row = r - 2 * w;
+                double *x = sU + row + ld * k;
+                double x0 = x[0], x1 = x[ld], x2 = x[2 * ld];
+                double t = tau * (x0 + v1 * x1 + v2 * x2);
+                x[0] = x0 - t; x[ld] = x1 - t * v1; x[2 * ld] = x2 - t * v2;
+            }
+        }
+        __syncthreads();
+        // ---------------- phase R: right reflectors ----------------
+        if (tid < nb) {
+            const int b = tid, kind = bk[b];
+            double *B = bv + 8 * b;
+            if (kind == 1 || kind == 3) {
+                const int k = bc[b] + 1;
+                double wv[3], tau2;
+                d_invhse3(sT + k + ld * k, ld, wv, tau2);
+                B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
+            } else if (kind == 2) {
+                const int l = w - 1;
+                double cs, sn, r;
+                d_lartg(sT[l + ld * l], sT[l + ld * (l - 1)], cs, sn, r);
+                sT[l + ld * l] = r; sT[l + ld * (l - 1)] = 0.0;
+                B[6] = cs; B[7] = sn;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nb * 3 * w; idx += blockDim.x) {
+            const int b = idx / (3 * w), r = idx % (3 * w);
+            const int kind = bk[b];
+            if (kind == 0) continue;
+            const double *B = bv + 8 * b;
+            if (kind == 2) {
+                const int l = w - 1;
+                const double cs = B[6], sn = B[7];
+                double *M; int row;
+                if (r < w) { M = sH; row = r; }
+                else if (r < 2 * w) { M = sT; row = r - w; if (row >= l) continue; }
+                else { M = sV; row = r - 2 * w; }
+                double x = M[row + ld * l], y = M[row + ld * (l - 1)];
+                M[row + ld * l] = cs * x + sn * y; M[row + ld * (l - 1)] = cs * y - sn * x;
+                continue;
+            }
+            const int k = bc[b] + 1;
+            const double w1 = B[3], w2 = B[4], tau2 = B[5];
+            if (tau2 == 0.0) continue;
+            double *M; int row;
+            if (r < w) { M = sH; row = r; if (row >= min(k + 4, w)) continue; }
+            else if (r < 2 * w) { M = sT; row = r - w; if (row >= k + 3) continue; }
+            else { M = sV; row = r - 2 * w; }
+            double *x = M + row + ld * k;
+            double x0 = x[0], x1 = x[ld], x2 = x[2 * ld];
+            double t = tau2 * (x0 + w1 * x1 + w2 * x2);
+            x[0] = x0 - t; x[ld] = x1 - t * w1; x[2 * ld] = x2 - t * w2;
+        }
+        __syncthreads();
+    }
+    // clean the exact zeros left by the last right reflector of every bulge, then the
+    // vigilant deflation check of DHGEQZ5.f:254-275 over the window
+    if (tid < nb) {
+        const int b = tid;
+        const int start = a.intro ? 3 * (nb - 1 - b) : 0;
+        const int nst = a.intro ? 3 * b + 1 + a.steps : a.steps;
+        const int cst = a.intro ? -1 : a.t0 + 3 * b;
+        int last_c = cst + min(nst, total - start) - 1;
+        int k = last_c + 1;
+        if (k + 2 <= w - 1 && k >= 0) { sT[(k + 1) + ld * k] = 0.0; sT[(k + 2) + ld * k] = 0.0; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int j = 1; j < w; ++j) {
+            double hj = sH[j + ld * (j - 1)];
+            if (hj == 0.0) continue;
+            double x = sH[(j - 1) + ld * (j - 1)], y = sH[j + ld * j];
+            if (fabs(hj) <= fmax(a.smlnum, GQZ_ULP * (fabs(x) + fabs(y)))) {
+                double up = sH[(j - 1) + ld * j];
+                double h12 = fmax(fabs(hj), fabs(up)), h21 = fmin(fabs(hj), fabs(up));
+                double h11 = fmax(fabs(y), fabs(x - y)), h22 = fmin(fabs(y), fabs(x - y));
+                double scl = h11 + h12, tst2 = h22 * (h11 / scl);
+                if (tst2 == 0.0 || h21 * (h12 / scl) <= fmax(a.smlnum, GQZ_ULP * tst2)) sH[j + ld * (j - 1)] = 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    double *oH = a.H + GQZ_IX(a.p, a.p, a.ld);
+    double *oT = a.T + GQZ_IX(a.p, a.p, a.ld);
+    for (int idx = tid; idx < w * w; idx += blockDim.x) {
+        int i = idx % w, j = idx / w;
+        oH[GQZ_IX(i, j, a.ld)] = sH[i + ld * j];
+        oT[GQZ_IX(i, j, a.ld)] = sT[i + ld * j];
+        a.U[i + (size_t)w * j] = sU[i + ld * j];
+        a.V[i + (size_t)w * j] = sV[i + ld * j];
+    }
+}
+
+int gqz_chase_max_window() { return CH_MAXW; }
+
+void gqz_chase_window(hipStream_t st, const GqzChainArgs &a) {
+    size_t lds = sizeof(double) * (4 * (size_t)(a.w + 1) * a.w + 8 * CH_MAXB) + sizeof(int) * 2 * CH_MAXB;
+    hipFuncSetAttribute((const void *)k_chase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_chase, dim3(1), dim3(256), lds, st, a);
+}

@@ -1,0 +1,797 @@
+// qz_host.cpp -- host side of the MI355X multishift QZ: the PDHGEQZ0/PDHGEQZ1 iteration
+// (deflation scan, infinite eigenvalues, aggressive early deflation, multishift sweeps) driving
+// the HIP kernels on one stream, and the C ABI declared in include/gqz.h.
+//
+// Reference structure (NLAFET/GEVP-PDHGEQZ, 1x1 grid):
+//   qz_run        ~ PDHGEQZ0.f:505-1202 / PDHGEQZ1.f:466-763 (recursive: the AED window is
+//                   solved by the same routine, PDHGEQZ3.f:588)
+//   small_block   ~ PDHGEQZ4 + KKQZ for blocks <= NMIN (PDHGEQZ4.f:14-405)
+//   aed           ~ PDHGEQZ3/PDHGEQZ2 + QZEARLY (PDHGEQZ3.f:330-997, qzearly.f:1-455)
+//   sweep         ~ PDHGEQZ5 with chains of SHIFTPERWND shifts (PDHGEQZ0.f:1068-1187)
+//   apply_window  ~ PDHGEQZ6 (PDHGEQZ6.f:276-380)
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gqz.h"
+#include "gqz_kernels.h"
+#include "gqz_misc.h"
+
+namespace {
+
+const double kUlp = 2.220446049250313080847e-16;
+const double kSafmin = 2.2250738585072013831e-308;
+
+struct GqzError : std::runtime_error {
+    explicit GqzError(const std::string &s) : std::runtime_error(s) {}
+};
+
+#define HIPCHK(x)                                                                               \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess)                                                                   \
+            throw GqzError(std::string(#x) + " failed: " + hipGetErrorString(e_));              \
+    } while (0)
+
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, top = 0;
+    double *alloc(size_t n) {
+        size_t b = (n * sizeof(double) + 255) & ~(size_t)255;
+        if (top + b > cap) throw GqzError("workspace exhausted");
+        double *p = (double *)(base + top);
+        top += b;
+        return p;
+    }
+    size_t mark() const { return top; }
+    void release(size_t m) { top = m; }
+};
+
+struct Lib {
+    int dev = -1;
+    hipStream_t st = nullptr;
+    Arena ws;
+    int *d_out = nullptr, *h_out = nullptr;
+    double *h_buf = nullptr;   // pinned staging (readbacks and shift uploads)
+    size_t h_buf_n = 0;
+    double *d_norm = nullptr;
+    gqz_policy_fn pol = nullptr;
+    void *pol_user = nullptr;
+    std::vector<gqz_trace_rec> trace;
+    int trace_cap = 0;
+    std::string err;
+    std::mutex mu;
+} L;
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int piparmq_ns(int nh) {            // piparmq.f:445-476
+    int ns = 2;
+    if (nh >= 30) ns = 4;
+    if (nh >= 60) ns = 10;
+    if (nh >= 150) {
+        int lg = (int)std::lround(std::log((double)nh) / std::log(2.0));
+        ns = std::max(10, nh / lg);
+    }
+    if (nh >= 590) ns = 64;
+    if (nh >= 3000) ns = 128;
+    if (nh >= 6000) ns = 256;
+    if (nh >= 12000) ns = 512;
+    if (nh >= 24000) ns = 1024;
+    if (nh >= 48000) ns = 2048;
+    return std::max(2, ns - ns % 2);
+}
+int piparmq_nw(int nh) { int ns = piparmq_ns(nh); return nh <= 500 ? ns : 3 * ns / 2; }
+int piparmq_nibble(int nh) {
+    if (nh < 6000) return 14;
+    return std::min(100, (int)std::ceil(335.0 * std::pow((double)nh, -0.44)));
+}
+
+struct View {
+    double *H, *T, *Q, *Z;
+    int ld, ldq, n, nq;
+    double *ar, *ai, *be;
+    double smlnum;
+};
+
+struct Run {
+    double ttol;
+    gqz_params prm;
+    gqz_stats st;
+    int n_top;
+    // deflation history for the parameter policy (top level)
+    std::vector<double> hist_nd, hist_frac;
+    int last_ns = 0, last_nw = 0, last_nibble = 0, last_swept = 0, last_ndinf = 0;
+};
+
+void sync_read(void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipStreamSynchronize(L.st));
+}
+
+// ---- PDHGEQZ6 role: apply the window's U, V outside the window [p, p+w) -------------------
+void apply_window(Run &R, const View &v, int p, int w, const double *U, const double *V) {
+    GqzSlabJobs Jl = {};
+    int ncol = v.n - p - w;
+    if (ncol > 0) {
+        Jl.njobs = 2;
+        Jl.C[0] = v.H + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[0] = v.ld; Jl.m[0] = ncol; Jl.F[0] = U; Jl.ldf[0] = w;
+        Jl.C[1] = v.T + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[1] = v.ld; Jl.m[1] = ncol; Jl.F[1] = U; Jl.ldf[1] = w;
+        gqz_slab_left_multi(L.st, Jl, w);
+        R.st.slab_launches++;
+    }
+    GqzSlabJobs Jr = {};
+    int q = 0;
+    if (p > 0) {
+        Jr.C[q] = v.H + (size_t)p * v.ld; Jr.ldc[q] = v.ld; Jr.m[q] = p; Jr.F[q] = V; Jr.ldf[q] = w; ++q;
+        Jr.C[q] = v.T + (size_t)p * v.ld; Jr.ldc[q] = v.ld; Jr.m[q] = p; Jr.F[q] = V; Jr.ldf[q] = w; ++q;
+    }
+    if (v.Q) { Jr.C[q] = v.Q + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = U; Jr.ldf[q] = w; ++q; }
+    if (v.Z) { Jr.C[q] = v.Z + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = V; Jr.ldf[q] = w; ++q; }
+    Jr.njobs = q;
+    if (q > 0) { gqz_slab_right_multi(L.st, Jr, w); R.st.slab_launches++; }
+}
+
+// ---- small active block: LDS-resident base QZ, then the off-block updates ------------------
+int small_block(Run &R, const View &v, int lo, int hi) {
+    double t0 = now();
+    int m = hi - lo + 1;
+    size_t mk = L.ws.mark();
+    double *U = L.ws.alloc((size_t)m * m), *V = L.ws.alloc((size_t)m * m);
+    GqzSmallJob J = {};
+    J.H = v.H + (size_t)lo + (size_t)lo * v.ld; J.T = v.T + (size_t)lo + (size_t)lo * v.ld;
+    J.ld = v.ld; J.m = m; J.U = U; J.V = V; J.ldu = m;
+    J.ar = v.ar + lo; J.ai = v.ai + lo; J.be = v.be + lo;
+    J.info = L.d_out + 8; J.ttol = R.ttol; J.smlnum = v.smlnum;
+    gqz_launch_small1(L.st, J);
+    apply_window(R, v, lo, m, U, V);
+    sync_read(L.h_out + 8, L.d_out + 8, sizeof(int));
+    int info = L.h_out[8];
+    L.ws.release(mk);
+    R.st.small_calls++;
+    R.st.t_small += now() - t0;
+    return info ? lo + info : 0;
+}
+
+int qz_run(Run &R, const View &v, int ilo, int ihi, int level);
+
+// ---- aggressive early deflation on the window [kbot-jw+1, kbot] ---------------------------
+int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vector<double> &sh, int &ls) {
+    double t0 = now();
+    int kwtop = kbot - jw + 1;
+    size_t mk = L.ws.mark();
+    size_t jj = (size_t)jw * jw;
+    double *Hw = L.ws.alloc(jj), *Tw = L.ws.alloc(jj), *U = L.ws.alloc(jj), *V = L.ws.alloc(jj);
+    double *weig = L.ws.alloc(3 * (size_t)jw), *scr = L.ws.alloc(jw);
+    gqz_launch_window_in(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, V);
+    View W;
+    W.H = Hw; W.T = Tw; W.Q = U; W.Z = V; W.ld = jw; W.ldq = jw; W.n = jw; W.nq = jw;
+    W.ar = weig; W.ai = weig + jw; W.be = weig + 2 * jw;
+    W.smlnum = kSafmin * ((double)jw / kUlp);
+    int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
+    int info;
+    if (jw <= nmin) {
+        GqzSmallJob J = {};
+        J.H = Hw; J.T = Tw; J.ld = jw; J.m = jw; J.U = U; J.V = V; J.ldu = jw;
+        J.ar = W.ar; J.ai = W.ai; J.be = W.be; J.info = L.d_out + 9; J.ttol = R.ttol; J.smlnum = W.smlnum;
+        gqz_launch_small1(L.st, J);
+        sync_read(L.h_out + 9, L.d_out + 9, sizeof(int));
+        info = L.h_out[9];
+    } else {
+        info = qz_run(R, W, 0, jw - 1, level + 1);
+    }
+    if (info != 0) {      // soft recovery: no deflation (PDHGEQZ0.f:944-949)
+        L.ws.release(mk);
+        ls = 0;
+        return 0;
+    }
+    const double *sptr = (kwtop > ktop) ? v.H + (size_t)kwtop + (size_t)(kwtop - 1) * v.ld : nullptr;
+    gqz_launch_aed_post(L.st, Hw, Tw, U, V, jw, sptr, W.smlnum, W.ar, W.ai, W.be, scr, L.d_out + 16);
+    HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 4 * sizeof(int), hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipStreamSynchronize(L.st));
+    int kbc = L.h_out[16];
+    R.st.swaps += L.h_out[17];
+    R.st.swap_fail += L.h_out[18] ? 1 : 0;
+    int spike = L.h_out[19];
+    if (kbc > 0 && spike) gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+    gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? 1 : 0);
+    apply_window(R, v, kwtop, jw, U, V);
+    sh.assign(3 * (size_t)kbc, 0.0);
+    for (int i = 0; i < kbc; ++i) {
+        sh[i] = L.h_buf[i];
+        sh[kbc + i] = L.h_buf[jw + i];
+        sh[2 * kbc + i] = L.h_buf[2 * jw + i];
+    }
+    ls = kbc;
+    L.ws.release(mk);
+    R.st.aed_calls++;
+    if (level == 0) R.st.t_aed += now() - t0;
+    return jw - kbc;
+}
+
+// ---- one chain of nb bulges from ktop to kbot (PDHGEQZ5 with NUMWIN = 1) -------------------
+void chase_chain(Run &R, const View &v, int ktop, int kbot, int nb, const double *sr, const double *si,
+                 const double *sb) {
+    const int WMAX = gqz_chase_max_window();
+    size_t mk = L.ws.mark();
+    double *U = L.ws.alloc((size_t)WMAX * WMAX), *V = L.ws.alloc((size_t)WMAX * WMAX);
+    GqzChainArgs a = {};
+    a.H = v.H; a.T = v.T; a.ld = v.ld; a.nb = nb; a.U = U; a.V = V; a.sr = sr; a.si = si; a.sb = sb;
+    a.smlnum = v.smlnum;
+    int p = ktop;
+    int w = std::min(WMAX, kbot - ktop + 1);
+    int last = (p + w - 1 == kbot);
+    a.p = p; a.w = w; a.intro = 1; a.t0 = -1; a.last = last;
+    a.steps = last ? w : (w - 3 * nb - 1);
+    gqz_chase_window(L.st, a);
+    apply_window(R, v, p, w, U, V);
+    R.st.chase_windows++;
+    int gtop = p + a.steps;
+    while (!last) {
+        p = gtop;
+        w = std::min(WMAX, kbot - p + 1);
+        last = (p + w - 1 == kbot);
+        a.p = p; a.w = w; a.intro = 0; a.t0 = 0; a.last = last;
+        a.steps = last ? w : (w - 3 * nb - 1);
+        gqz_chase_window(L.st, a);
+        apply_window(R, v, p, w, U, V);
+        R.st.chase_windows++;
+        gtop = p + a.steps;
+    }
+    L.ws.release(mk);
+}
+
+// shift preparation (PDHGEQZ0.f:1068-1135) and the chains of the sweep
+void sweep(Run &R, const View &v, int ktop, int kbot, std::vector<double> &sh, int avail, int ns_cap,
+           int level) {
+    double t0 = now();
+    std::vector<double> sr(sh.begin(), sh.begin() + avail), si(sh.begin() + avail, sh.begin() + 2 * avail),
+        sb(sh.begin() + 2 * avail, sh.begin() + 3 * avail);
+    int ns = std::min(ns_cap, std::max(2, kbot - ktop - 1));
+    ns -= ns % 2;
+    if (avail < 2) {
+        // too few shifts: eigenvalues of the bottom 2x2 (exceptional fall-back)
+        double hb[4], tb[4];
+        for (int j = 0; j < 2; ++j) {
+            HIPCHK(hipMemcpyAsync(hb + 2 * j, v.H + (size_t)(kbot - 1) + (size_t)(kbot - 1 + j) * v.ld, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(tb + 2 * j, v.T + (size_t)(kbot - 1) + (size_t)(kbot - 1 + j) * v.ld, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st));
+        }
+        HIPCHK(hipStreamSynchronize(L.st));
+        double m10 = hb[1] / tb[3], m11 = hb[3] / tb[3];
+        double m00 = (hb[0] - tb[2] * m10) / tb[0], m01 = (hb[2] - tb[2] * m11) / tb[0];
+        double tr = 0.5 * (m00 + m11), det = m00 * m11 - m01 * m10, disc = tr * tr - det;
+        sr.assign(2, tr); si.assign(2, 0.0); sb.assign(2, 1.0);
+        if (disc >= 0) { sr[0] = tr + std::sqrt(disc); sr[1] = tr - std::sqrt(disc); }
+        else { si[0] = std::sqrt(-disc); si[1] = -si[0]; }
+        avail = 2;
+    }
+    if (avail > ns) {        // bubble sort by decreasing |lambda|
+        for (int k = avail - 1; k > 0; --k) {
+            bool sorted = true;
+            for (int i = 0; i < k; ++i) {
+                double d1 = std::fabs((std::fabs(sr[i]) + std::fabs(si[i])) / sb[i]);
+                double d2 = std::fabs((std::fabs(sr[i + 1]) + std::fabs(si[i + 1])) / sb[i + 1]);
+                if (d1 < d2) { std::swap(sr[i], sr[i + 1]); std::swap(si[i], si[i + 1]); std::swap(sb[i], sb[i + 1]); sorted = false; }
+            }
+            if (sorted) break;
+        }
+    }
+    for (int i = avail - 1; i >= 2; i -= 2) {   // keep conjugate pairs together
+        if (si[i] == 0.0 && si[i - 1] != 0.0) {
+            double t;
+            t = sr[i]; sr[i] = sr[i - 1]; sr[i - 1] = sr[i - 2]; sr[i - 2] = t;
+            t = si[i]; si[i] = si[i - 1]; si[i - 1] = si[i - 2]; si[i - 2] = t;
+            t = sb[i]; sb[i] = sb[i - 1]; sb[i - 1] = sb[i - 2]; sb[i - 2] = t;
+        }
+    }
+    ns = std::min(ns, avail);
+    ns -= ns % 2;
+    if (ns < 2) return;
+    int ks = avail - ns;
+    // non-finite shifts are replaced by a harmless real pair (the bottom-most finite one)
+    for (int i = ks; i < avail; ++i)
+        if (!std::isfinite(sr[i] / sb[i]) || !std::isfinite(si[i] / sb[i])) { sr[i] = 1.0; si[i] = 0.0; sb[i] = 1.0; }
+    for (int i = ks; i + 1 < avail; i += 2)
+        if ((si[i] == 0.0) != (si[i + 1] == 0.0)) { si[i] = si[i + 1] = 0.0; }
+    // upload [sr | si | sb] of the ns shifts used
+    size_t mk = L.ws.mark();
+    double *d_sh = L.ws.alloc(3 * (size_t)ns);
+    for (int i = 0; i < ns; ++i) {
+        L.h_buf[i] = sr[ks + i]; L.h_buf[ns + i] = si[ks + i]; L.h_buf[2 * ns + i] = sb[ks + i];
+    }
+    HIPCHK(hipMemcpyAsync(d_sh, L.h_buf, 3 * sizeof(double) * ns, hipMemcpyHostToDevice, L.st));
+    int chain = R.prm.chain > 0 ? R.prm.chain : 20;
+    chain = std::max(2, std::min(chain, 40));
+    chain -= chain % 2;
+    for (int off = 0; off < ns; off += chain) {
+        int cnt = std::min(chain, ns - off);
+        cnt -= cnt % 2;
+        if (cnt < 2) break;
+        chase_chain(R, v, ktop, kbot, cnt / 2, d_sh + off, d_sh + ns + off, d_sh + 2 * ns + off);
+    }
+    // the staging buffer is reused by the next readback: make sure the upload has completed
+    HIPCHK(hipStreamSynchronize(L.st));
+    L.ws.release(mk);
+    R.st.sweeps++;
+    R.st.shifts_used += ns;
+    if (level == 0) R.st.t_sweep += now() - t0;
+}
+
+// deflation-history features for the parameter policy (12 values, all O(1))
+void features(Run &R, int nh, float f[12]) {
+    int n = std::max(1, R.n_top);
+    auto last = [](const std::vector<double> &h, int k) {
+        return (int)h.size() > k ? h[h.size() - 1 - k] : 0.0;
+    };
+    double m3 = 0.0;
+    int c3 = 0;
+    for (int k = 0; k < 3 && k < (int)R.hist_frac.size(); ++k) { m3 += last(R.hist_frac, k); ++c3; }
+    if (c3) m3 /= c3;
+    f[0] = (float)(std::log2((double)std::max(nh, 2)) / 16.0);
+    f[1] = (float)((double)nh / n);
+    f[2] = (float)last(R.hist_frac, 0);
+    f[3] = (float)last(R.hist_frac, 1);
+    f[4] = (float)m3;
+    f[5] = (float)((double)R.last_ndinf / std::max(1, nh));
+    f[6] = (float)((double)R.last_ns / std::max(1, nh));
+    f[7] = (float)((double)R.last_nw / std::max(1, nh));
+    f[8] = (float)(R.last_nibble / 100.0);
+    f[9] = (float)R.last_swept;
+    f[10] = (float)(std::min<double>(R.hist_frac.size(), 200) / 200.0);
+    f[11] = (float)(std::log2((double)std::max(n, 2)) / 16.0);
+}
+
+int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
+    int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
+    int nh0 = ihi - ilo + 1;
+    if (nh0 <= 0) return 0;
+    if (nh0 <= nmin) return small_block(R, v, ilo, ihi);
+    const bool top = (level == 0);
+    int ns_def = (top && R.prm.ns > 0) ? R.prm.ns : piparmq_ns(nh0);
+    int nw_def = (top && R.prm.nw > 0) ? R.prm.nw : piparmq_nw(nh0);
+    int nib_def = (top && R.prm.nibble > 0) ? R.prm.nibble : piparmq_nibble(nh0);
+    long maxit = (long)(R.prm.maxit > 0 ? R.prm.maxit : 30) * nh0;
+    std::vector<double> sh;
+    int kbot = ihi;
+    for (long it = 0;; ++it) {
+        if (kbot < ilo) break;
+        if (it >= maxit) return kbot + 1;
+        double ti = now();
+        R.st.iters++;
+        gqz_launch_inf_defl(L.st, v.H, v.T, v.Q, v.Z, v.ld, v.ldq, v.n, v.nq, ilo, kbot, R.ttol, v.smlnum, v.ar,
+                            v.ai, v.be, L.d_out);
+        sync_read(L.h_out, L.d_out, 4 * sizeof(int));
+        R.st.t_inf += now() - ti;
+        int ktop = L.h_out[0], kt = L.h_out[1], kb = L.h_out[2], ninf = L.h_out[3];
+        if (ninf > 0) {
+            R.st.inf_defl += ninf;
+            if (top) R.last_ndinf = ninf;
+            if (kt > ktop && kb >= kt) {
+                int r = qz_run(R, v, kt, kb, level);
+                if (r) return r;
+                kbot = ktop - 1;
+            } else {
+                kbot = (kb < kt) ? ktop - 1 : kb;
+            }
+            continue;
+        }
+        int nh = kbot - ktop + 1;
+        if (nh <= nmin) {
+            int r = small_block(R, v, ktop, kbot);
+            if (r) return r;
+            kbot = ktop - 1;
+            continue;
+        }
+        int ns_it = ns_def, nw_it = nw_def, nib_it = nib_def;
+        float feat[12];
+        if (top) {
+            features(R, nh, feat);
+            if (R.prm.use_policy && L.pol) {
+                int a = -1, b = -1, c = -1;
+                L.pol(L.pol_user, feat, nh, &a, &b, &c);
+                if (a > 0) ns_it = std::max(2, a - a % 2);
+                if (b > 0) nw_it = std::max(4, b);
+                if (c > 0) nib_it = c;
+            }
+        }
+        nw_it = std::min(nw_it, 1024);
+        int jw = std::min(nw_it, nh);
+        int ls = 0;
+        int nd = aed(R, v, ktop, kbot, jw, level, sh, ls);
+        kbot -= nd;
+        int swept = 0;
+        if (kbot >= ktop && (nd == 0 || (100 * nd <= jw * nib_it && kbot - ktop + 1 > nmin))) {
+            sweep(R, v, ktop, kbot, sh, ls, ns_it, level);
+            swept = 1;
+        }
+        if (top) {
+            R.hist_frac.push_back((double)nd / std::max(1, jw));
+            R.last_ns = ns_it; R.last_nw = nw_it; R.last_nibble = nib_it; R.last_swept = swept;
+            if (L.trace_cap > 0 && (int)L.trace.size() < L.trace_cap) {
+                gqz_trace_rec rec;
+                rec.nh = nh; rec.ktop = ktop; rec.kbot = kbot + nd; rec.ns = ns_it; rec.nw = nw_it;
+                rec.nibble = nib_it; rec.nd = nd; rec.ndinf = 0; rec.swept = swept;
+                std::memcpy(rec.feat, feat, sizeof feat);
+                rec.t_iter = now() - ti;
+                L.trace.push_back(rec);
+            }
+            R.last_ndinf = 0;
+        }
+    }
+    return 0;
+}
+
+size_t arena_bytes(int n, const gqz_params *prm) {
+    int nw = piparmq_nw(std::max(n, 1));
+    if (prm && prm->nw > nw) nw = prm->nw;
+    nw = std::max(std::min(std::max(nw, 1024), n), 64);
+    size_t d = 6 * (size_t)nw * nw + 4 * (size_t)1024 * 1024;
+    return d * sizeof(double);
+}
+
+void ensure_ws(size_t bytes) {
+    if (L.ws.cap >= bytes) return;
+    if (L.ws.base) HIPCHK(hipFree(L.ws.base));
+    L.ws.base = nullptr; L.ws.cap = 0; L.ws.top = 0;
+    HIPCHK(hipMalloc((void **)&L.ws.base, bytes));
+    L.ws.cap = bytes;
+}
+
+int check_init() {
+    if (L.dev < 0) {
+        if (gqz_init(0) != 0) return -1;
+    }
+    HIPCHK(hipSetDevice(L.dev));
+    return 0;
+}
+
+int decode(char c, const char *opts) {
+    char u = (char)toupper((unsigned char)c);
+    for (int i = 0; opts[i]; ++i) if (opts[i] == u) return i;
+    return -1;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+extern "C" {
+
+const char *gqz_version(void) { return "gqz 0.1 (gfx950, fp64)"; }
+const char *gqz_last_error(void) { return L.err.c_str(); }
+void *gqz_stream(void) { return (void *)L.st; }
+
+int gqz_init(int device) {
+    try {
+        std::lock_guard<std::mutex> g(L.mu);
+        int cnt = 0;
+        if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) {
+            L.err = "no HIP device available: the MI355X QZ library requires a GPU";
+            return -1;
+        }
+        if (device < 0 || device >= cnt) { L.err = "bad device index"; return -1; }
+        if (L.dev == device) return 0;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+        HIPCHK(hipMalloc((void **)&L.d_out, 64 * sizeof(int)));
+        HIPCHK(hipMalloc((void **)&L.d_norm, 8 * sizeof(double)));
+        HIPCHK(hipHostMalloc((void **)&L.h_out, 64 * sizeof(int), hipHostMallocDefault));
+        L.h_buf_n = 3 * 8192 + 64;
+        HIPCHK(hipHostMalloc((void **)&L.h_buf, L.h_buf_n * sizeof(double), hipHostMallocDefault));
+        L.dev = device;
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -1;
+    }
+}
+
+int gqz_finalize(void) {
+    if (L.dev < 0) return 0;
+    hipSetDevice(L.dev);
+    hipStreamSynchronize(L.st);
+    if (L.ws.base) hipFree(L.ws.base);
+    L.ws = Arena();
+    hipFree(L.d_out); hipFree(L.d_norm); hipHostFree(L.h_out); hipHostFree(L.h_buf);
+    hipStreamDestroy(L.st);
+    L.st = nullptr; L.dev = -1;
+    return 0;
+}
+
+int gqz_sync(void) {
+    if (L.dev < 0) return 0;
+    return hipStreamSynchronize(L.st) == hipSuccess ? 0 : -1;
+}
+
+int gqz_set_policy(gqz_policy_fn fn, void *user) { L.pol = fn; L.pol_user = user; return 0; }
+int gqz_trace_enable(int cap) { L.trace.clear(); L.trace_cap = std::max(0, cap); return 0; }
+int gqz_trace_get(gqz_trace_rec *out, int cap) {
+    int k = std::min<int>(cap, (int)L.trace.size());
+    if (out) for (int i = 0; i < k; ++i) out[i] = L.trace[i];
+    return k;
+}
+
+int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi, double *H, int ldh, double *T,
+                  int ldt, double *alphar, double *alphai, double *beta, double *Q, int ldq, double *Z, int ldz,
+                  const gqz_params *prm, gqz_stats *stats) {
+    // argument checks in PDHGEQZ order (PDHGEQZ.f:344-405)
+    int ijob = decode(job, "ES"), icq = decode(compq, "NIV"), icz = decode(compz, "NIV");
+    if (ijob < 0) return -1;
+    if (icq < 0) return -2;
+    if (icz < 0) return -3;
+    if (n < 0) return -4;
+    if (ilo < 1 || ilo > std::max(1, n)) return -5;
+    if (ihi < std::min(ilo, n) || ihi > n) return -6;
+    if (ldh < std::max(1, n) || ldt != ldh) return -8;
+    if (icq > 0 && (ldq < std::max(1, n) || !Q)) return -15;
+    if (icz > 0 && (ldz < std::max(1, n) || !Z)) return -17;
+    if (icq > 0 && icz > 0 && ldq != ldz) return -17;
+    if (n == 0) return 0;
+    try {
+        if (check_init()) return -100;
+        double t0 = now();
+        Run R;
+        R.prm = prm ? *prm : gqz_params{};
+        std::memset(&R.st, 0, sizeof R.st);
+        R.n_top = n;
+        ensure_ws(arena_bytes(n, prm));
+        L.ws.top = 0;
+        if (L.trace_cap > 0) L.trace.clear();
+        // TTOL = ulp * ||T(ilo:ihi, ilo:ihi)||_F (PDHGEQZ.f:456-458)
+        HIPCHK(hipMemsetAsync(L.d_norm, 0, sizeof(double), L.st));
+        gqz_launch_sumsq(L.st, T + (size_t)(ilo - 1) * (ldt + 1), ldt, ihi - ilo + 1, ihi - ilo + 1, 1, L.d_norm);
+        double tf2 = 0.0;
+        sync_read(&tf2, L.d_norm, sizeof(double));
+        R.ttol = std::max(kSafmin, kUlp * std::sqrt(tf2));
+        if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
+        if (icz == 1) gqz_launch_identity(L.st, Z, ldz, n, n);
+        gqz_launch_fill(L.st, alphar + (ilo - 1), ihi - ilo + 1, -1.0);
+        gqz_launch_fill(L.st, alphai + (ilo - 1), ihi - ilo + 1, -1.0);
+        gqz_launch_fill(L.st, beta + (ilo - 1), ihi - ilo + 1, -1.0);
+        View v;
+        v.H = H; v.T = T; v.Q = icq ? Q : nullptr; v.Z = icz ? Z : nullptr;
+        v.ld = ldh; v.ldq = icq ? ldq : ldz; v.n = n; v.nq = n;
+        v.ar = alphar; v.ai = alphai; v.be = beta;
+        v.smlnum = kSafmin * ((double)(ihi - ilo + 1) / kUlp);
+        int info = qz_run(R, v, ilo - 1, ihi - 1, 0);
+        // eigenvalues (and beta >= 0) from the final Schur form, including the parts outside
+        // [ilo, ihi] which are read off the diagonal
+        gqz_launch_finalize(L.st, H, T, v.Z, ldh, v.ldq, n, 0, n - 1, alphar, alphai, beta);
+        HIPCHK(hipStreamSynchronize(L.st));
+        R.st.t_total = now() - t0;
+        if (stats) *stats = R.st;
+        return info;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_geqrb_dev(char compq, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq) {
+    int icq = decode(compq, "NIV");
+    if (icq < 0) return -1;
+    if (n < 0) return -2;
+    if (n == 0) return 0;
+    try {
+        if (check_init()) return -100;
+        double *tau = nullptr;
+        HIPCHK(hipMallocAsync((void **)&tau, sizeof(double) * (size_t)n, L.st));
+        if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
+        gqz_launch_house_qr(L.st, n, A, lda, B, ldb, icq ? Q : nullptr, ldq, tau);
+        HIPCHK(hipFreeAsync(tau, L.st));
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, int lda, double *B, int ldb,
+                  double *Q, int ldq, double *Z, int ldz) {
+    int icq = decode(compq, "NIV"), icz = decode(compz, "NIV");
+    if (icq < 0) return -1;
+    if (icz < 0) return -2;
+    if (n < 0) return -3;
+    if (ilo < 1 || ilo > std::max(1, n)) return -4;
+    if (ihi < std::min(ilo, n) || ihi > n) return -5;
+    if (lda < std::max(1, n) || ldb != lda) return -7;
+    if ((icq > 0 && icz > 0) && ldq != ldz) return -13;
+    if (n == 0) return 0;
+    try {
+        if (check_init()) return -100;
+        if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
+        if (icz == 1) gqz_launch_identity(L.st, Z, ldz, n, n);
+        gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
+                            icq ? ldq : ldz, n);
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_compres_dev(int n, const double *A, const double *B, const double *S, const double *T, const double *Q,
+                    const double *Z, int ld, double out[4]) {
+    if (n <= 0) return -1;
+    try {
+        if (check_init()) return -100;
+        double *W1 = nullptr, *W2 = nullptr;
+        size_t nn = (size_t)n * n;
+        HIPCHK(hipMallocAsync((void **)&W1, nn * sizeof(double), L.st));
+        HIPCHK(hipMallocAsync((void **)&W2, nn * sizeof(double), L.st));
+        HIPCHK(hipMemsetAsync(L.d_norm, 0, 8 * sizeof(double), L.st));
+        const double *X[2] = { A, B }, *Y[2] = { S, T };
+        for (int t = 0; t < 2; ++t) {
+            gqz_gemm(L.st, 0, 0, n, n, n, X[t], ld, Z, ld, W1, n);        // X Z
+            gqz_gemm(L.st, 1, 0, n, n, n, Q, ld, W1, n, W2, n);           // Q^T X Z
+            gqz_launch_sub(L.st, W2, n, Y[t], ld, n);
+            gqz_launch_sumsq(L.st, W2, n, n, n, 0, L.d_norm + 2 * t);
+            gqz_launch_sumsq(L.st, X[t], ld, n, n, 0, L.d_norm + 2 * t + 1);
+        }
+        const double *O[2] = { Q, Z };
+        for (int t = 0; t < 2; ++t) {
+            gqz_gemm(L.st, 1, 0, n, n, n, O[t], ld, O[t], ld, W2, n);
+            gqz_launch_sub(L.st, W2, n, nullptr, n, n);
+            gqz_launch_sumsq(L.st, W2, n, n, n, 0, L.d_norm + 4 + t);
+        }
+        double h[8];
+        HIPCHK(hipMemcpyAsync(h, L.d_norm, 8 * sizeof(double), hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipFreeAsync(W1, L.st));
+        HIPCHK(hipFreeAsync(W2, L.st));
+        HIPCHK(hipStreamSynchronize(L.st));
+        out[0] = std::sqrt(h[0]) / std::sqrt(h[1]);
+        out[1] = std::sqrt(h[2]) / std::sqrt(h[3]);
+        out[2] = std::sqrt(h[4]) / (n * kUlp);
+        out[3] = std::sqrt(h[5]) / (n * kUlp);
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_slab_update_dev(char side, int m, int k, double *Cm, int ldc, const double *F, int ldf) {
+    try {
+        if (check_init()) return -100;
+        char s = (char)toupper((unsigned char)side);
+        if (s == 'R') gqz_slab_right(L.st, Cm, ldc, m, k, F, ldf, nullptr);
+        else if (s == 'L') gqz_slab_left(L.st, Cm, ldc, k, m, F, ldf, nullptr);
+        else return -1;
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_chase_window_dev(double *H, double *T, int ld, int p, int w, int nb, int t0, int steps, int intro,
+                         int last, double *U, double *V, const double *sr, const double *si, const double *sb) {
+    if (w < 4 || w > gqz_chase_max_window() || nb < 1 || nb > 20) return -1;
+    try {
+        if (check_init()) return -100;
+        GqzChainArgs a = {};
+        a.H = H; a.T = T; a.ld = ld; a.p = p; a.w = w; a.nb = nb; a.t0 = t0; a.steps = steps; a.intro = intro;
+        a.last = last; a.U = U; a.V = V; a.sr = sr; a.si = si; a.sb = sb;
+        a.smlnum = kSafmin * ((double)w / kUlp);
+        gqz_chase_window(L.st, a);
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V, double *alphar, double *alphai,
+                     double *beta, double ttol) {
+    if (m < 1 || m > 64) return -1;
+    try {
+        if (check_init()) return -100;
+        GqzSmallJob J = {};
+        J.H = H; J.T = T; J.ld = ld; J.m = m; J.U = U; J.V = V; J.ldu = m;
+        J.ar = alphar; J.ai = alphai; J.be = beta; J.info = L.d_out + 10; J.ttol = ttol;
+        J.smlnum = kSafmin * ((double)m / kUlp);
+        gqz_launch_small1(L.st, J);
+        sync_read(L.h_out + 10, L.d_out + 10, sizeof(int));
+        return L.h_out[10];
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+// ---- host-pointer entry points: copy in, run, copy out ---------------------------------------
+static int host_roundtrip_hgeqz(char job, char compq, char compz, int n, int ilo, int ihi, double *H, int ldh,
+                                double *T, int ldt, double *alphar, double *alphai, double *beta, double *Q,
+                                int ldq, double *Z, int ldz, const gqz_params *prm, gqz_stats *stats) {
+    if (n <= 0) return gqz_hgeqz_dev(job, compq, compz, n, ilo, ihi, H, ldh, T, ldt, alphar, alphai, beta, Q,
+                                     ldq, Z, ldz, prm, stats);
+    int icq = decode(compq, "NIV"), icz = decode(compz, "NIV");
+    if (icq < 0) return -2;
+    if (icz < 0) return -3;
+    if (ldh < n) return -8;
+    if (ldt < n) return -10;
+    try {
+        if (check_init()) return -100;
+        size_t nn = (size_t)n * n;
+        double *d = nullptr;
+        HIPCHK(hipMalloc((void **)&d, sizeof(double) * (4 * nn + 3 * (size_t)n)));
+        double *dH = d, *dT = d + nn, *dQ = d + 2 * nn, *dZ = d + 3 * nn, *de = d + 4 * nn;
+        HIPCHK(hipMemcpy2DAsync(dH, n * sizeof(double), H, ldh * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpy2DAsync(dT, n * sizeof(double), T, ldt * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icq == 2) HIPCHK(hipMemcpy2DAsync(dQ, n * sizeof(double), Q, ldq * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icz == 2) HIPCHK(hipMemcpy2DAsync(dZ, n * sizeof(double), Z, ldz * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        int info = gqz_hgeqz_dev(job, compq, compz, n, ilo, ihi, dH, n, dT, n, de, de + n, de + 2 * n,
+                                 icq ? dQ : nullptr, n, icz ? dZ : nullptr, n, prm, stats);
+        if (info >= 0) {
+            HIPCHK(hipMemcpy2DAsync(H, ldh * sizeof(double), dH, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpy2DAsync(T, ldt * sizeof(double), dT, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icq) HIPCHK(hipMemcpy2DAsync(Q, ldq * sizeof(double), dQ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(alphar, de, n * sizeof(double), hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(alphai, de + n, n * sizeof(double), hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(beta, de + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, L.st));
+        }
+        HIPCHK(hipStreamSynchronize(L.st));
+        HIPCHK(hipFree(d));
+        return info;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_hgeqz(char job, char compq, char compz, int n, int ilo, int ihi, double *H, int ldh, double *T, int ldt,
+              double *alphar, double *alphai, double *beta, double *Q, int ldq, double *Z, int ldz,
+              const gqz_params *prm, gqz_stats *stats) {
+    return host_roundtrip_hgeqz(job, compq, compz, n, ilo, ihi, H, ldh, T, ldt, alphar, alphai, beta, Q, ldq, Z,
+                                ldz, prm, stats);
+}
+
+int gqz_gghrd(char compq, char compz, int n, int ilo, int ihi, double *A, int lda, double *B, int ldb, double *Q,
+              int ldq, double *Z, int ldz) {
+    int icq = decode(compq, "NIV"), icz = decode(compz, "NIV");
+    if (icq < 0) return -1;
+    if (icz < 0) return -2;
+    if (n <= 0) return n < 0 ? -3 : 0;
+    try {
+        if (check_init()) return -100;
+        size_t nn = (size_t)n * n;
+        double *d = nullptr;
+        HIPCHK(hipMalloc((void **)&d, sizeof(double) * 4 * nn));
+        double *dA = d, *dB = d + nn, *dQ = d + 2 * nn, *dZ = d + 3 * nn;
+        HIPCHK(hipMemcpy2DAsync(dA, n * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpy2DAsync(dB, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icq == 2) HIPCHK(hipMemcpy2DAsync(dQ, n * sizeof(double), Q, ldq * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icz == 2) HIPCHK(hipMemcpy2DAsync(dZ, n * sizeof(double), Z, ldz * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        int info = gqz_gghrd_dev(compq, compz, n, ilo, ihi, dA, n, dB, n, dQ, n, dZ, n);
+        if (info == 0) {
+            HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpy2DAsync(B, ldb * sizeof(double), dB, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icq) HIPCHK(hipMemcpy2DAsync(Q, ldq * sizeof(double), dQ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+        }
+        HIPCHK(hipStreamSynchronize(L.st));
+        HIPCHK(hipFree(d));
+        return info;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+}  // extern "C"

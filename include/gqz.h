@@ -1,0 +1,133 @@
+/*
+ * gqz.h -- C ABI of the MI355X multishift QZ (libgqz.so).
+ *
+ * Drop-in boundary for the PDHGEQZ / PDGGHRD path of NLAFET/GEVP-PDHGEQZ on a 1x1 process grid.
+ * All matrices are column-major FP64.  "_dev" entry points take device (HBM) pointers and run
+ * on the library's HIP stream; the host-pointer entry points copy in, run, and copy out.
+ * Indices in this header are 1-based where they mirror a Fortran argument (ILO, IHI) and
+ * 0-based nowhere else.  No torch types cross this boundary.
+ */
+#ifndef GQZ_H
+#define GQZ_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tuning of one call.  A zero field selects the PIPARMQ default for the active block size
+ * (src/pdhgeqz/piparmq.f:445-517: NS table, NW = 3NS/2 above 500, NIBBLE). */
+typedef struct {
+    int nmin;      /* blocks <= nmin use the LDS-resident base QZ (<= 64)                     */
+    int ns;        /* shifts per sweep (ISPEC 53)                                               */
+    int nw;        /* AED window (ISPEC 51)                                                     */
+    int nibble;    /* sweep-skip threshold in percent (ISPEC 52)                                */
+    int chain;     /* shifts per bulge chain (SHIFTPERWND, PDHGEQZ0.f:1145-1148); <= 40         */
+    int maxit;     /* iteration cap factor (MAXIT = maxit * NH, 30 in the reference)           */
+    int use_policy;/* 1: consult the installed parameter policy at every top-level iteration  */
+} gqz_params;
+
+typedef struct {
+    long iters, aed_calls, sweeps, shifts_used, small_calls, swaps, swap_fail, inf_defl;
+    long chase_windows, slab_launches;
+    double t_total, t_aed, t_sweep, t_small, t_inf;   /* seconds (host-timed, stream-synced) */
+} gqz_stats;
+
+/* Per top-level iteration record (the learned-parameter trainer's input). */
+typedef struct {
+    int nh, ktop, kbot, ns, nw, nibble, nd, ndinf, swept;
+    float feat[12];
+    double t_iter;
+} gqz_trace_rec;
+
+/* Parameter policy: called at each top-level iteration with the 12 deflation-history features;
+ * writes ns, nw, nibble (<= 0 keeps the default). */
+typedef void (*gqz_policy_fn)(void *user, const float feat[12], int nh, int *ns, int *nw, int *nibble);
+
+/* ---- library ---------------------------------------------------------------------------- */
+int gqz_init(int device);                 /* select the device, create stream + workspace    */
+int gqz_finalize(void);
+const char *gqz_version(void);
+void *gqz_stream(void);                   /* hipStream_t used by the _dev entry points        */
+int gqz_sync(void);
+int gqz_set_policy(gqz_policy_fn fn, void *user);
+int gqz_trace_enable(int cap);            /* record up to cap iterations (0 disables)         */
+int gqz_trace_get(gqz_trace_rec *out, int cap);   /* returns the number of records          */
+const char *gqz_last_error(void);
+
+/* ---- QZ ----------------------------------------------------------------------------------
+ * Replaces PDHGEQZ (src/pdhgeqz/PDHGEQZ.f:15-31) for a 1x1 grid.
+ * job: 'E' or 'S'; compq, compz: 'N', 'I', 'V' (PDHGEQZ.f:132-161).  ilo, ihi are 1-based.
+ * On exit H = Q^T H_in Z (Schur form), T likewise, alphar/alphai/beta per PDHGEQZ.f:51-66
+ * (beta >= 0, beta = 0 exactly for infinite eigenvalues).  Returns INFO (PDHGEQZ.f:496-502):
+ * < 0 argument error, > 0 no convergence (index of the unconverged bottom), 0 success. */
+int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi,
+                  double *H, int ldh, double *T, int ldt,
+                  double *alphar, double *alphai, double *beta,
+                  double *Q, int ldq, double *Z, int ldz,
+                  const gqz_params *prm, gqz_stats *stats);
+int gqz_hgeqz(char job, char compq, char compz, int n, int ilo, int ihi,
+              double *H, int ldh, double *T, int ldt,
+              double *alphar, double *alphai, double *beta,
+              double *Q, int ldq, double *Z, int ldz,
+              const gqz_params *prm, gqz_stats *stats);
+
+/* ---- HT reduction --------------------------------------------------------------------------
+ * Replaces PDGGHRD (src/ab2ht/PDGGHRD.f:13-14).  B must be upper triangular on entry.
+ * Q/Z convention here is PDHGEQZ's (A_out = Q^T A_in Z); the Fortran shim transposes Q to
+ * PDGGHRD's own convention (PDGGHRD.f:25-27). */
+int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, int lda,
+                  double *B, int ldb, double *Q, int ldq, double *Z, int ldz);
+int gqz_gghrd(char compq, char compz, int n, int ilo, int ihi, double *A, int lda,
+              double *B, int ldb, double *Q, int ldq, double *Z, int ldz);
+/* QR of B applied to A (and accumulated into Q): the pre-step of the drivers
+ * (testing/EXRAND2.f:296-329; PDGEQRF + PDORMQR). compq: 'N', 'I', 'V'. */
+int gqz_geqrb_dev(char compq, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq);
+
+/* ---- ScaLAPACK-descriptor surface for the Fortran shim (1x1 grid) ----------------------------
+ * desc[9] = DTYPE, CTXT, M, N, MB, NB, RSRC, CSRC, LLD (PDHGEQZ.f:88-127).  Host pointers.
+ * Workspace query when ldwork == -1 or liwork == -1 (PDHGEQZ.f:409-430). */
+int gqz_pdhgeqz_host(char job, char compq, char compz, int n, int ilo, int ihi,
+                     double *H, const int *desch, double *T, const int *desct,
+                     double *alphar, double *alphai, double *beta,
+                     double *Q, const int *descq, double *Z, const int *descz,
+                     double *dwork, int ldwork, int *iwork, int liwork);
+int gqz_pdgghrd_host(char compq, char compz, int n, int ilo, int ihi,
+                     double *A, const int *desca, double *B, const int *descb,
+                     double *Q, const int *descq, double *Z, const int *descz,
+                     double *work, int lwork);
+
+/* ---- residual check (PCOMPRES, tools/PQZHLPS.f:19-319) on device data ------------------------
+ * out[0] = ||S - Q^T A Z||_F/||A||_F, out[1] likewise for B, out[2] = ||I - Q^T Q||_F/(n eps),
+ * out[3] likewise for Z. */
+int gqz_compres_dev(int n, const double *A, const double *B, const double *S, const double *T,
+                    const double *Q, const double *Z, int ld, double out[4]);
+
+/* ---- kernel-level entry points (parity tests of single kernels) --------------------------------
+ * Slab update (the PDHGEQZ6 role): side 'R': C[m x k] <- C F ; side 'L': C[k x m] <- F^T C. */
+int gqz_slab_update_dev(char side, int m, int k, double *C, int ldc, const double *F, int ldf);
+/* One window of the chain chase (PDHGEQZB + DHGEQZ5 on a window): the w x w window at (p, p) of
+ * H, T (ld) holds nb bulges with tail columns t0 + 3b (intro = 1: introduce them first);
+ * every bulge advances `steps` columns (last = 1: push off at the window bottom).  U, V (w x w,
+ * ld w) receive the window's accumulated transforms; shifts are 2*nb (sr, si, sb) on the device.
+ * Only the window is modified. */
+int gqz_chase_window_dev(double *H, double *T, int ld, int p, int w, int nb, int t0, int steps,
+                         int intro, int last, double *U, double *V, const double *sr,
+                         const double *si, const double *sb);
+/* LDS-resident base QZ of an m x m block (m <= 64) at (H, T) with U, V (m x m, ld m) and the
+ * block's eigenvalues; returns the kernel's INFO. */
+int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,
+                     double *alphar, double *alphai, double *beta, double ttol);
+
+/* ---- batched pencils (no counterpart in the reference) ---------------------------------------
+ * count pencils of orders n[i] stored back to back (column-major, ld = n[i]) at offsets off[i]
+ * (in doubles) of the device arrays A and B; full pipeline QR(B) + HT + QZ per pencil in one
+ * persistent launch.  Q, Z (same layout) may be null.  Eigenvalues at eoff[i]. */
+int gqz_batched_dev(int count, const int *n, const long *off, const long *eoff,
+                    double *A, double *B, double *Q, double *Z,
+                    double *alphar, double *alphai, double *beta, int *info, int wg_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
