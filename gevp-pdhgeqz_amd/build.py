@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("GQZ_FC", "/opt/rocm/bin/amdflang")
 ARCH = os.environ.get("GQZ_ARCH", "gfx950")
 
-HIP_SOURCES = ["k_wgqz.hip", "k_misc.hip", "k_gemm.hip", "k_chase.hip", "k_ht.hip", "k_batched.hip",
+HIP_SOURCES = ["k_wgqz.hip", "k_aed.hip", "k_misc.hip", "k_gemm.hip", "k_chase.hip", "k_ht.hip", "k_batched.hip",
                "k_mlp.hip"]
 CXX_SOURCES = ["qz_host.cpp", "qz_desc.cpp"]
 

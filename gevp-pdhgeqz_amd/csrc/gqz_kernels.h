@@ -19,9 +19,6 @@ size_t gqz_small_lds_bytes(int m);
 #ifdef __HIPCC__
 extern "C" __global__ void k_small_qz(const GqzSmallJob *jobs);
 extern "C" __global__ void k_small_qz1(GqzSmallJob job);
-extern "C" __global__ void k_aed_post(double *Hw, double *Tw, double *U, double *V, int jw,
-                                      const double *sptr, double smlnum, double *war, double *wai,
-                                      double *wb, double *scr, int *out);
 extern "C" __global__ void k_gghrd_wg(double *A, double *B, int ld, int n, int ilo, int ihi,
                                       double *Q, double *Z, int ldq, int nq);
 #endif

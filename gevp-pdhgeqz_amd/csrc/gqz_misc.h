@@ -14,9 +14,6 @@ void gqz_launch_fill(hipStream_t st, double *x, int n, double v);
 void gqz_launch_finalize(hipStream_t st, double *H, double *T, double *Z, int ld, int ldz, int nz, int ilo,
                          int ihi, double *ar, double *ai, double *be);
 void gqz_launch_small(hipStream_t st, const GqzSmallJob *djobs, int njobs, int mmax);
-void gqz_launch_aed_post(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
-                         const double *sptr, double smlnum, double *war, double *wai, double *wb,
-                         double *scr, int *out);
 void gqz_launch_small1(hipStream_t st, const GqzSmallJob &job);
 void gqz_launch_gghrd_wg(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
                          double *Z, int ldq, int nq);
@@ -24,3 +21,21 @@ void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, i
                          double *tau);
 void gqz_launch_sumsq(hipStream_t st, const double *A, int ld, int m, int n, int upper, double *out);
 void gqz_launch_sub(hipStream_t st, double *R, int ldr, const double *S, int lds, int n);
+// AED window post-processing (k_aed.hip); pscr holds (jw/2 + 1) * 65 doubles
+void gqz_launch_aed_reorder(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
+                            const double *sptr, double smlnum, double *pscr, int *out);
+void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
+                          const double *sptr, double *war, double *wai, double *wb, double *scr, const int *out);
+
+// library state shared with entry points living beside their kernels (qz_host.cpp)
+int gqz_init_current();                 // initialise on device 0 if needed and make it current; 0 = ok
+void gqz_set_error(const char *msg);    // text returned by gqz_last_error
+
+#include <stdexcept>
+#include <string>
+#define GQZ_HIP(x)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess)                                                                   \
+            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));          \
+    } while (0)

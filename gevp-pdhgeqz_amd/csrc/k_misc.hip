@@ -220,12 +220,6 @@ void gqz_launch_small(hipStream_t st, const GqzSmallJob *djobs, int njobs, int m
     hipLaunchKernelGGL(k_small_qz, dim3(njobs), dim3(256), lds, st, djobs);
 }
 size_t gqz_small_lds_bytes(int m) { return sizeof(double) * 4 * (size_t)(m + 1) * m + sizeof(WgCtl) + 64; }
-void gqz_launch_aed_post(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
-                         const double *sptr, double smlnum, double *war, double *wai, double *wb,
-                         double *scr, int *out) {
-    hipLaunchKernelGGL(k_aed_post, dim3(1), dim3(1024), 0, st, Hw, Tw, U, V, jw, sptr, smlnum, war, wai, wb, scr,
-                       out);
-}
 void gqz_launch_small1(hipStream_t st, const GqzSmallJob &job) {
     size_t lds = gqz_small_lds_bytes(job.m);
     hipFuncSetAttribute((const void *)k_small_qz1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -3,293 +3,12 @@
 //  * k_small_qz   : double-shift QZ of a small active block held entirely in LDS (the
 //                   KKQZ / QZLAP role for blocks <= 64, PDHGEQZ4.f:14-405 semantics); emits the
 //                   block's Schur form, the accumulated U, V and its eigenvalues.
-//  * k_aed_post   : AED deflation test + reordering + spike reflector + RQ of T on a window
-//                   resident in HBM (QZEARLY, src/kkqz/qzearly.f:250-444; PDHGEQZ3.f:630-935).
 //  * k_gghrd_wg   : Givens Hessenberg-triangular reduction of a pencil by one workgroup
 //                   (PDGGHRD semantics, used to restore HT form inside AED, PDHGEQZ3.f:964).
 //
 // One lane computes each small transform; the workgroup applies it (see gqz_wg.h).
-#include "gqz_wg.h"
+#include "gqz_ds.h"
 #include "gqz_kernels.h"
-
-enum { OP_NOP = 0, OP_SEGDONE, OP_COLROT, OP_ROWROT, OP_G2, OP_NEG, OP_ZCHASE, OP_SWEEP, OP_FAIL };
-enum { POST_NONE = 0, POST_G2 };
-
-// --------------------------------------------------------------------------------------------
-// double-shift sweep on [f, l] (intro at f, 3x3 chase, push-off at l): DHGEQZ5.f:128-251,
-// PDHGEQZA.f:290-330, PDHGEQZB.f:368-406.
-// --------------------------------------------------------------------------------------------
-__device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, double smlnum) {
-    const int tid = threadIdx.x;
-    for (int k = f; k <= l - 2; ++k) {
-        if (tid == 0) {
-            double *H = P.H, *T = P.T;
-            int ld = P.ld;
-            double v0, v1, v2, tau;
-            if (k == f) {
-                double v[3];
-                d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
-                            C->sb[0], C->sr[1], C->sb[1], v);
-                v0 = v[0]; v1 = v[1]; v2 = v[2];
-                d_larfg3(v0, v1, v2, tau);
-            } else {
-                T[GQZ_IX(k, k - 1, ld)] = 0.0; T[GQZ_IX(k + 1, k - 1, ld)] = 0.0;
-                v0 = H[GQZ_IX(k, k - 1, ld)]; v1 = H[GQZ_IX(k + 1, k - 1, ld)]; v2 = H[GQZ_IX(k + 2, k - 1, ld)];
-                double beta = v0;
-                d_larfg3(beta, v1, v2, tau);
-                if (H[GQZ_IX(k + 2, k - 1, ld)] != 0.0 || H[GQZ_IX(k + 2, k, ld)] != 0.0 ||
-                    H[GQZ_IX(k + 2, k + 1, ld)] == 0.0) {
-                    H[GQZ_IX(k, k - 1, ld)] = beta;
-                } else {
-                    // collapsed bulge: try to re-introduce it (DHGEQZ5.f:141-189)
-                    double w[3];
-                    d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
-                                C->sb[0], C->sr[1], C->sb[1], w);
-                    double a1 = w[0], x1 = w[1], x2 = w[2], tau1;
-                    d_larfg3(a1, x1, x2, tau1);
-                    double refsum = tau1 * (H[GQZ_IX(k, k - 1, ld)] + x1 * H[GQZ_IX(k + 1, k - 1, ld)]);
-                    if (fabs(H[GQZ_IX(k + 1, k - 1, ld)] - refsum * x1) + fabs(refsum * x2) >
-                        GQZ_ULP * (fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]) +
-                                   fabs(H[GQZ_IX(k + 1, k + 1, ld)]))) {
-                        H[GQZ_IX(k, k - 1, ld)] = beta;
-                    } else {
-                        H[GQZ_IX(k, k - 1, ld)] -= refsum;
-                        tau = tau1; v1 = x1; v2 = x2;
-                    }
-                }
-                H[GQZ_IX(k + 1, k - 1, ld)] = 0.0; H[GQZ_IX(k + 2, k - 1, ld)] = 0.0;
-            }
-            C->v[0] = 1.0; C->v[1] = v1; C->v[2] = v2; C->tau = tau;
-        }
-        __syncthreads();
-        wg_left3(P, k, k, k, P.n, C->v, C->tau);
-        __syncthreads();
-        if (tid == 0) {
-            double w[3], tau2;
-            d_invhse3(P.T + GQZ_IX(k, k, P.ld), P.ld, w, tau2);
-            C->w[0] = 1.0; C->w[1] = w[1]; C->w[2] = w[2]; C->tau2 = tau2;
-        }
-        __syncthreads();
-        wg_right3(P, k, 0, min(k + 4, l + 1), k + 3, C->w, C->tau2);
-        __syncthreads();
-    }
-    // push-off at the bottom
-    if (tid == 0) {
-        double *H = P.H, *T = P.T;
-        int ld = P.ld;
-        T[GQZ_IX(l - 1, l - 2, ld)] = 0.0; T[GQZ_IX(l, l - 2, ld)] = 0.0;
-        double c, s, r;
-        d_lartg(H[GQZ_IX(l - 1, l - 2, ld)], H[GQZ_IX(l, l - 2, ld)], c, s, r);
-        H[GQZ_IX(l - 1, l - 2, ld)] = r; H[GQZ_IX(l, l - 2, ld)] = 0.0;
-        C->c = c; C->s = s;
-    }
-    __syncthreads();
-    wg_rowrot_all(P, l - 1, l - 1, l - 1, C->c, C->s);
-    __syncthreads();
-    if (tid == 0) {
-        double *T = P.T;
-        int ld = P.ld;
-        double c, s, r;
-        d_lartg(T[GQZ_IX(l, l, ld)], T[GQZ_IX(l, l - 1, ld)], c, s, r);
-        T[GQZ_IX(l, l, ld)] = r; T[GQZ_IX(l, l - 1, ld)] = 0.0;
-        C->c = c; C->s = s;
-    }
-    __syncthreads();
-    wg_colrot_all(P, l, l - 1, l + 1, l, C->c, C->s);
-    __syncthreads();
-    if (tid == 0) {
-        // vigilant deflation (DHGEQZ5.f:254-275)
-        double *H = P.H;
-        int ld = P.ld;
-        for (int j = f + 1; j <= l; ++j) {
-            double hj = H[GQZ_IX(j, j - 1, ld)];
-            if (hj == 0.0) continue;
-            double a = H[GQZ_IX(j - 1, j - 1, ld)], b = H[GQZ_IX(j, j, ld)];
-            if (fabs(hj) <= fmax(smlnum, GQZ_ULP * (fabs(a) + fabs(b)))) {
-                double up = H[GQZ_IX(j - 1, j, ld)];
-                double h12 = fmax(fabs(hj), fabs(up)), h21 = fmin(fabs(hj), fabs(up));
-                double h11 = fmax(fabs(b), fabs(a - b)), h22 = fmin(fabs(b), fabs(a - b));
-                double scl = h11 + h12, tst2 = h22 * (h11 / scl);
-                if (tst2 == 0.0 || h21 * (h12 / scl) <= fmax(smlnum, GQZ_ULP * tst2))
-                    H[GQZ_IX(j, j - 1, ld)] = 0.0;
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// lane-0 decision step of the base QZ; fills C->op and its parameters.
-__device__ void ds_decide(const WgPencil &P, WgCtl *C, double *ar, double *ai, double *be,
-                          double ttol, double smlnum) {
-    double *H = P.H, *T = P.T;
-    int ld = P.ld;
-    C->op = OP_NOP;
-    if (C->ilast < C->lo) { C->op = OP_SEGDONE; return; }
-    if (++C->its > C->maxit) { C->op = OP_FAIL; C->info = C->ilast + 1; return; }
-    int lo = C->lo, il = C->ilast;
-    int f = lo;
-    for (int j = il; j > lo; --j) {
-        double tst = fabs(H[GQZ_IX(j - 1, j - 1, ld)]) + fabs(H[GQZ_IX(j, j, ld)]);
-        if (fabs(H[GQZ_IX(j, j - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) {
-            H[GQZ_IX(j, j - 1, ld)] = 0.0; f = j; break;
-        }
-    }
-    C->ifirst = f;
-    // zeros on diag(T) (PDHGEQZ7.f:380)
-    double tol = fmax(ttol, smlnum);
-    int found = -1;
-    for (int i = il; i >= f; --i)
-        if (fabs(T[GQZ_IX(i, i, ld)]) <= tol) { T[GQZ_IX(i, i, ld)] = 0.0; if (found < 0) found = i; }
-    if (found >= 0) {
-        C->iiter = 0;
-        if (il == f) {
-            ar[il] = H[GQZ_IX(il, il, ld)]; ai[il] = 0.0; be[il] = 0.0;
-            C->ilast = il - 1; return;
-        }
-        double c, s, r;
-        if (T[GQZ_IX(il, il, ld)] == 0.0) {
-            d_lartg(H[GQZ_IX(il, il, ld)], H[GQZ_IX(il, il - 1, ld)], c, s, r);
-            H[GQZ_IX(il, il, ld)] = r; H[GQZ_IX(il, il - 1, ld)] = 0.0;
-            ar[il] = r; ai[il] = 0.0; be[il] = 0.0;
-            C->op = OP_COLROT; C->i0 = il; C->i1 = il - 1; C->i2 = il; C->i3 = il; C->c = c; C->s = s;
-            C->ilast = il - 1; return;
-        }
-        if (T[GQZ_IX(f, f, ld)] == 0.0) {
-            d_lartg(H[GQZ_IX(f, f, ld)], H[GQZ_IX(f + 1, f, ld)], c, s, r);
-            H[GQZ_IX(f, f, ld)] = r; H[GQZ_IX(f + 1, f, ld)] = 0.0;
-            ar[f] = r; ai[f] = 0.0; be[f] = 0.0;
-            C->op = OP_ROWROT; C->i0 = f; C->i1 = f + 1; C->i2 = f + 1; C->c = c; C->s = s;
-            if (f > lo) { C->stk_lo[C->sp] = lo; C->stk_hi[C->sp] = f - 1; C->sp++; }
-            C->lo = f + 1; return;
-        }
-        C->op = OP_ZCHASE; C->i0 = found; C->i1 = f; C->i2 = il; return;
-    }
-    if (f == il) {
-        if (T[GQZ_IX(il, il, ld)] < 0.0) { C->op = OP_NEG; C->i0 = il; }
-        ar[il] = H[GQZ_IX(il, il, ld)]; ai[il] = 0.0; be[il] = T[GQZ_IX(il, il, ld)];
-        if (C->op == OP_NEG) { ar[il] = -ar[il]; be[il] = -be[il]; }
-        C->ilast = il - 1; C->iiter = 0; return;
-    }
-    if (f == il - 1) {
-        double A[4] = { H[GQZ_IX(f, f, ld)], H[GQZ_IX(f + 1, f, ld)], H[GQZ_IX(f, f + 1, ld)], H[GQZ_IX(f + 1, f + 1, ld)] };
-        double B[4] = { T[GQZ_IX(f, f, ld)], 0.0, T[GQZ_IX(f, f + 1, ld)], T[GQZ_IX(f + 1, f + 1, ld)] };
-        int cplx = d_lagv2(A, B, C->G, C->G + 4);
-        for (int q = 0; q < 4; ++q) { C->S[q] = A[q]; C->Tm[q] = B[q]; }
-        C->op = OP_G2; C->i0 = f; C->i1 = cplx;
-        if (cplx) {
-            double a2[2], i2[2], b2[2];
-            d_pair_eigs(A, B, a2, i2, b2);
-            ar[f] = a2[0]; ai[f] = i2[0]; be[f] = b2[0];
-            ar[f + 1] = a2[1]; ai[f + 1] = i2[1]; be[f + 1] = b2[1];
-            C->ilast = il - 2;
-        }
-        C->iiter = 0;
-        return;
-    }
-    // double-shift sweep on [f, il]
-    C->iiter++;
-    double er[2], ei[2];
-    d_eig22(H[GQZ_IX(il - 1, il - 1, ld)], H[GQZ_IX(il, il - 1, ld)], H[GQZ_IX(il - 1, il, ld)],
-            H[GQZ_IX(il, il, ld)], T[GQZ_IX(il - 1, il - 1, ld)], T[GQZ_IX(il - 1, il, ld)],
-            T[GQZ_IX(il, il, ld)], er, ei);
-    if (C->iiter % 10 == 0) {
-        double e = fabs(H[GQZ_IX(il, il - 1, ld)] / T[GQZ_IX(il - 1, il - 1, ld)]) +
-                   fabs(H[GQZ_IX(il - 1, il - 2, ld)] / T[GQZ_IX(il - 2, il - 2, ld)]);
-        double lam = H[GQZ_IX(il, il, ld)] / T[GQZ_IX(il, il, ld)] + 0.75 * e;
-        er[0] = er[1] = lam; ei[0] = ei[1] = 0.0;
-        if (C->iiter % 20 == 0) er[1] = -0.4375 * e + H[GQZ_IX(il, il, ld)] / T[GQZ_IX(il, il, ld)];
-    }
-    if (!isfinite(er[0]) || !isfinite(er[1]) || !isfinite(ei[0])) { er[0] = er[1] = 1.0; ei[0] = ei[1] = 0.0; }
-    C->sr[0] = er[0]; C->sr[1] = er[1]; C->si[0] = ei[0]; C->si[1] = ei[1]; C->sb[0] = C->sb[1] = 1.0;
-    C->op = OP_SWEEP; C->i0 = f; C->i1 = il;
-}
-
-// interior zero of diag(T) chased to the bottom (DHGEQZ7 / PDHGEQZ8 direction)
-__device__ void wg_zchase(const WgPencil &P, WgCtl *C, int j, int kt, int kb) {
-    const int tid = threadIdx.x;
-    for (int jch = j; jch < kb; ++jch) {
-        if (tid == 0) {
-            double c, s, r;
-            d_lartg(P.T[GQZ_IX(jch, jch + 1, P.ld)], P.T[GQZ_IX(jch + 1, jch + 1, P.ld)], c, s, r);
-            P.T[GQZ_IX(jch, jch + 1, P.ld)] = r; P.T[GQZ_IX(jch + 1, jch + 1, P.ld)] = 0.0;
-            C->c = c; C->s = s;
-        }
-        __syncthreads();
-        wg_rowrot_all(P, jch, jch > kt ? jch - 1 : jch, jch + 2, C->c, C->s);
-        __syncthreads();
-        if (jch > kt) {
-            if (tid == 0) {
-                double c, s, r;
-                d_lartg(P.H[GQZ_IX(jch + 1, jch, P.ld)], P.H[GQZ_IX(jch + 1, jch - 1, P.ld)], c, s, r);
-                P.H[GQZ_IX(jch + 1, jch, P.ld)] = r; P.H[GQZ_IX(jch + 1, jch - 1, P.ld)] = 0.0;
-                C->c = c; C->s = s;
-            }
-            __syncthreads();
-            wg_colrot_all(P, jch, jch - 1, jch + 1, jch, C->c, C->s);
-            __syncthreads();
-        }
-    }
-}
-
-// Base QZ of the pencil's block [ilo, ihi] with full-row/column updates inside the pencil and
-// U, V accumulation.  Returns INFO (0 = converged).
-__device__ int wg_qz_ds(const WgPencil &P, WgCtl *C, int ilo, int ihi, double *ar, double *ai,
-                        double *be, double ttol, double smlnum) {
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        C->sp = 0; C->info = 0; C->its = 0; C->maxit = 40 * (ihi - ilo + 1) + 20;
-        C->stk_lo[0] = ilo; C->stk_hi[0] = ihi; C->sp = 1;
-        C->op = OP_NOP;
-    }
-    __syncthreads();
-    for (;;) {
-        if (tid == 0) {
-            if (C->sp == 0 || C->info) C->op = OP_SEGDONE;
-            else { C->sp--; C->lo = C->stk_lo[C->sp]; C->ilast = C->stk_hi[C->sp]; C->iiter = 0; C->op = OP_NOP; }
-        }
-        __syncthreads();
-        if (C->op == OP_SEGDONE) break;
-        for (;;) {
-            if (tid == 0) ds_decide(P, C, ar, ai, be, ttol, smlnum);
-            __syncthreads();
-            int op = C->op;
-            if (op == OP_SEGDONE || op == OP_FAIL) break;
-            switch (op) {
-            case OP_COLROT: wg_colrot_all(P, C->i0, C->i1, C->i2, C->i3, C->c, C->s); break;
-            case OP_ROWROT: wg_rowrot_all(P, C->i0, C->i1, C->i2, C->c, C->s); break;
-            case OP_NEG: {
-                int j = C->i0;
-                for (int idx = tid; idx < 2 * (j + 1) + (P.V ? P.nq : 0); idx += blockDim.x) {
-                    if (idx <= j) P.H[GQZ_IX(idx, j, P.ld)] = -P.H[GQZ_IX(idx, j, P.ld)];
-                    else if (idx < 2 * (j + 1)) P.T[GQZ_IX(idx - j - 1, j, P.ld)] = -P.T[GQZ_IX(idx - j - 1, j, P.ld)];
-                    else P.V[GQZ_IX(idx - 2 * (j + 1), j, P.ldu)] = -P.V[GQZ_IX(idx - 2 * (j + 1), j, P.ldu)];
-                }
-                break;
-            }
-            case OP_G2: {
-                wg_g2_all(P, C->i0, C->i0 + 2, C->i0, C->G, C->G + 4);
-                __syncthreads();
-                if (tid == 0) {
-                    int f = C->i0;
-                    P.H[GQZ_IX(f, f, P.ld)] = C->S[0]; P.H[GQZ_IX(f + 1, f, P.ld)] = C->i1 ? C->S[1] : 0.0;
-                    P.H[GQZ_IX(f, f + 1, P.ld)] = C->S[2]; P.H[GQZ_IX(f + 1, f + 1, P.ld)] = C->S[3];
-                    P.T[GQZ_IX(f, f, P.ld)] = C->Tm[0]; P.T[GQZ_IX(f + 1, f, P.ld)] = 0.0;
-                    P.T[GQZ_IX(f, f + 1, P.ld)] = C->Tm[2]; P.T[GQZ_IX(f + 1, f + 1, P.ld)] = C->Tm[3];
-                }
-                break;
-            }
-            case OP_ZCHASE: wg_zchase(P, C, C->i0, C->i1, C->i2); break;
-            case OP_SWEEP: wg_ds_sweep(P, C, C->i0, C->i1, smlnum); break;
-            default: break;
-            }
-            __syncthreads();
-        }
-        if (C->op == OP_FAIL) break;
-    }
-    __syncthreads();
-    return C->info;
-}
-
 // --------------------------------------------------------------------------------------------
 // k_small_qz: one job per workgroup; the block is copied into LDS (ld = m + 1)
 // --------------------------------------------------------------------------------------------
@@ -325,206 +44,6 @@ extern "C" __global__ void __launch_bounds__(256) k_small_qz(const GqzSmallJob *
 extern "C" __global__ void __launch_bounds__(256) k_small_qz1(GqzSmallJob J) { small_qz_body(J); }
 
 // --------------------------------------------------------------------------------------------
-// k_aed_post: window (Hw, Tw, U, V) jw x jw in HBM after the window QZ.
-//   out[0] = kbc (rows 0..kbc-1 undeflatable / unexamined), out[1] = swaps, out[2] = failed
-//   eigenvalues of the whole window in war/wai/wb (window order after reordering); scr holds
-//   jw doubles of scratch for the reflector vectors.
-// --------------------------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(1024) k_aed_post(double *Hw, double *Tw, double *U,
-                                                              double *V, int jw, const double *sptr,
-                                                              double smlnum, double *war,
-                                                              double *wai, double *wb, double *scr,
-                                                              int *out) {
-    __shared__ WgCtl Cs;
-    WgCtl *C = &Cs;
-    const int tid = threadIdx.x;
-    const double s = sptr ? *sptr : 0.0;
-    WgPencil P;
-    P.H = Hw; P.T = Tw; P.U = U; P.V = V; P.ld = jw; P.ldu = jw; P.n = jw; P.nq = jw;
-    if (tid == 0) { C->i0 = 0; C->i1 = jw - 1; C->i2 = 0; C->i3 = 0; C->i4 = 0; C->info = 0; C->its = 0; }
-    __syncthreads();
-    // ---- deflation test from the bottom; undeflatable blocks moved to the top -------------
-    for (;;) {
-        if (tid == 0) {
-            // i0 = ilst, i1 = kb, i2 = knt, i3 = swaps
-            C->op = OP_NOP;
-            int ilst = C->i0, kb = C->i1;
-            if (C->i2 >= jw || kb < ilst) C->op = OP_SEGDONE;
-            else {
-                int bulge = (kb > 0) && Hw[GQZ_IX(kb, kb - 1, jw)] != 0.0;
-                int dfl;
-                if (!bulge) {
-                    C->i2 += 1;
-                    double tmp = fabs(s) + fabs(Hw[GQZ_IX(kb, kb, jw)]);
-                    dfl = fabs(s * U[GQZ_IX(0, kb, jw)]) <= fmax(GQZ_ULP * tmp, smlnum);
-                } else {
-                    C->i2 += 2;
-                    double h00 = Hw[GQZ_IX(kb - 1, kb - 1, jw)], h10 = Hw[GQZ_IX(kb, kb - 1, jw)];
-                    double h01 = Hw[GQZ_IX(kb - 1, kb, jw)], h11 = Hw[GQZ_IX(kb, kb, jw)];
-                    double rtdet;
-                    if (h00 == 0.0 && h10 == 0.0) rtdet = 0.0;
-                    else if (fabs(h00) >= fabs(h10)) rtdet = sqrt(fabs(h00)) * sqrt(fabs(h11 - (h10 / h00) * h01));
-                    else rtdet = sqrt(fabs(h10)) * sqrt(fabs(h01 - (h00 / h10) * h11));
-                    double tmp = fabs(s) + rtdet;
-                    dfl = fmax(fabs(s * U[GQZ_IX(0, kb, jw)]), fabs(s * U[GQZ_IX(0, kb - 1, jw)])) <=
-                          fmax(GQZ_ULP * tmp, smlnum);
-                }
-                if (dfl) { C->i1 = kb - (bulge ? 2 : 1); }
-                else {
-                    // move the block starting at here = kb - bulge up to ilst
-                    C->op = OP_SWEEP;           // reused tag: "reorder"
-                    C->i4 = kb - bulge;         // here
-                    C->ifirst = bulge ? 2 : 1;  // nbf
-                    C->lo = ilst;               // target
-                }
-            }
-        }
-        __syncthreads();
-        if (C->op == OP_SEGDONE) break;
-        if (C->op != OP_SWEEP) continue;
-        // ---- DTGEXC loop: adjacent swaps until the block sits at the target ----------------
-        for (;;) {
-            if (tid == 0) {
-                int here = C->i4, nbf = C->ifirst, target = C->lo;
-                C->op = OP_NOP;
-                if (here <= target) C->op = OP_SEGDONE;
-                else {
-                    int nbn = (here >= 2 && Hw[GQZ_IX(here - 1, here - 2, jw)] != 0.0) ? 2 : 1;
-                    int j1 = here - nbn, m = nbn + nbf;
-                    for (int jj = 0; jj < m; ++jj)
-                        for (int ii = 0; ii < m; ++ii) {
-                            C->S[ii + m * jj] = Hw[GQZ_IX(j1 + ii, j1 + jj, jw)];
-                            C->Tm[ii + m * jj] = Tw[GQZ_IX(j1 + ii, j1 + jj, jw)];
-                        }
-                    if (d_tgex2(m, nbn, nbf, C->S, C->Tm, C->Qa, C->Za)) {
-                        C->op = OP_FAIL;
-                    } else {
-                        C->op = OP_G2; C->i2 += 0; C->ilast = j1; C->iiter = m;
-                        C->i3++;
-                        C->i4 = here - nbn;
-                    }
-                }
-            }
-            __syncthreads();
-            if (C->op == OP_SEGDONE || C->op == OP_FAIL) break;
-            wg_gm_all(P, C->ilast, C->iiter, C->Qa, C->Za);
-            __syncthreads();
-            if (tid == 0) {
-                int j1 = C->ilast, m = C->iiter;
-                for (int jj = 0; jj < m; ++jj)
-                    for (int ii = 0; ii < m; ++ii) {
-                        Hw[GQZ_IX(j1 + ii, j1 + jj, jw)] = C->S[ii + m * jj];
-                        Tw[GQZ_IX(j1 + ii, j1 + jj, jw)] = C->Tm[ii + m * jj];
-                    }
-                // a 2x2 block that split into two 1x1 blocks continues as 1x1 blocks: the
-                // lower one is left in place and examined again by the deflation loop
-                if (C->ifirst == 2 && Hw[GQZ_IX(C->i4 + 1, C->i4, jw)] == 0.0) C->ifirst = 1;
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            if (C->op == OP_FAIL) { C->info = 1; }
-            else {
-                // the moved block now sits at the target: it is no longer "unexamined"
-                int nbf = C->ifirst;
-                C->i0 += nbf;
-                // re-examine the same bottom position (blocks above shifted down)
-            }
-        }
-        __syncthreads();
-        if (C->info) break;
-    }
-    // ---- eigenvalues of the window blocks (1x1 signs normalised later by the host pass) ----
-    if (tid == 0) {
-        for (int j = 0; j < jw;) {
-            if (j + 1 < jw && Hw[GQZ_IX(j + 1, j, jw)] != 0.0) {
-                double A[4] = { Hw[GQZ_IX(j, j, jw)], Hw[GQZ_IX(j + 1, j, jw)], Hw[GQZ_IX(j, j + 1, jw)], Hw[GQZ_IX(j + 1, j + 1, jw)] };
-                double B[4] = { Tw[GQZ_IX(j, j, jw)], 0.0, Tw[GQZ_IX(j, j + 1, jw)], Tw[GQZ_IX(j + 1, j + 1, jw)] };
-                double er[2], ei[2];
-                d_eig22(A[0], A[1], A[2], A[3], B[0], B[2], B[3], er, ei);
-                double b0 = B[0], b1 = B[3];
-                if (ei[0] != 0.0) {
-                    war[j] = er[0] * b0; wai[j] = fabs(ei[0]) * b0; wb[j] = b0;
-                    war[j + 1] = er[0] * b1; wai[j + 1] = -fabs(ei[0]) * b1; wb[j + 1] = b1;
-                } else {
-                    war[j] = er[0] * b0; wai[j] = 0.0; wb[j] = b0;
-                    war[j + 1] = er[1] * b1; wai[j + 1] = 0.0; wb[j + 1] = b1;
-                }
-                j += 2;
-            } else {
-                war[j] = Hw[GQZ_IX(j, j, jw)]; wai[j] = 0.0; wb[j] = Tw[GQZ_IX(j, j, jw)];
-                j += 1;
-            }
-        }
-        int kbc = C->i1 + 1;
-        if (kbc < C->i0) kbc = C->i0;
-        if (C->info) kbc = C->i1 + 1;
-        C->i4 = kbc;
-        out[0] = kbc; out[1] = C->i3; out[2] = C->info; out[3] = (s != 0.0);
-    }
-    __syncthreads();
-    const int kbc = C->i4;
-    if (kbc <= 0 || s == 0.0) return;
-    // ---- spike reflector on rows 0..kbc-1 (qzearly.f:369-386) -----------------------------
-    double *vsp = scr;        // jw doubles of scratch
-    if (tid == 0) {
-        for (int i = 0; i < kbc; ++i) vsp[i] = s * U[GQZ_IX(0, i, jw)];
-        double alpha = vsp[0], tau;
-        d_larfg(kbc, alpha, vsp + 1, 1, tau);
-        vsp[0] = 1.0;
-        C->tau = tau;
-    }
-    __syncthreads();
-    {
-        double tau = C->tau;
-        // left on Hw and Tw rows 0..kbc-1 (all columns), right on U columns 0..kbc-1
-        for (int idx = tid; idx < 3 * jw; idx += blockDim.x) {
-            if (idx < 2 * jw) {
-                double *M = idx < jw ? Hw : Tw;
-                int col = idx < jw ? idx : idx - jw;
-                double *p = M + GQZ_IX(0, col, jw);
-                double t = 0.0;
-                for (int i = 0; i < kbc; ++i) t += vsp[i] * p[i];
-                t *= tau;
-                for (int i = 0; i < kbc; ++i) p[i] -= t * vsp[i];
-            } else {
-                int row = idx - 2 * jw;
-                double t = 0.0;
-                for (int j = 0; j < kbc; ++j) t += U[GQZ_IX(row, j, jw)] * vsp[j];
-                t *= tau;
-                for (int j = 0; j < kbc; ++j) U[GQZ_IX(row, j, jw)] -= t * vsp[j];
-            }
-        }
-    }
-    __syncthreads();
-    // ---- RQ of Tw(0:kbc, 0:kbc), bottom row first; applied to Hw and V (qzearly.f:387-392) ---
-    for (int r = kbc - 1; r >= 1; --r) {
-        if (tid == 0) {
-            for (int j = 0; j < r; ++j) vsp[j] = Tw[GQZ_IX(r, j, jw)];
-            double alpha = Tw[GQZ_IX(r, r, jw)], tau;
-            d_larfg(r + 1, alpha, vsp, 1, tau);
-            vsp[r] = 1.0;
-            Tw[GQZ_IX(r, r, jw)] = alpha;
-            for (int j = 0; j < r; ++j) Tw[GQZ_IX(r, j, jw)] = 0.0;
-            C->tau = tau;
-        }
-        __syncthreads();
-        double tau = C->tau;
-        for (int idx = tid; idx < r + 2 * jw; idx += blockDim.x) {
-            double *M; int row;
-            if (idx < r) { M = Tw; row = idx; }
-            else if (idx < r + jw) { M = Hw; row = idx - r; }
-            else { M = V; row = idx - r - jw; }
-            double t = 0.0;
-            for (int j = 0; j <= r; ++j) t += M[GQZ_IX(row, j, jw)] * vsp[j];
-            t *= tau;
-            for (int j = 0; j <= r; ++j) M[GQZ_IX(row, j, jw)] -= t * vsp[j];
-        }
-        __syncthreads();
-    }
-}
-
-// --------------------------------------------------------------------------------------------
 // k_gghrd_wg: Givens HT reduction of A (active rows/cols [ilo, ihi]) with B upper triangular,
 // rows of A, B updated over columns up to n, columns over rows [0, ihi]; Q, Z (nq rows).
 // Each rotation pair costs two barriers: lane 0 computes the row rotation and, from the
@@ -535,81 +54,5 @@ extern "C" __global__ void __launch_bounds__(1024) k_gghrd_wg(double *A, double 
                                                                int ilo, int ihi, double *Q,
                                                                double *Z, int ldq, int nq) {
     __shared__ double sh[4];
-    const int tid = threadIdx.x;
-    for (int j = ilo; j <= ihi - 2; ++j) {
-        for (int i = ihi; i >= j + 2; --i) {
-            if (tid == 0) {
-                double c, s, r, cc, ss;
-                d_lartg(A[GQZ_IX(i - 1, j, ld)], A[GQZ_IX(i, j, ld)], c, s, r);
-                A[GQZ_IX(i - 1, j, ld)] = r; A[GQZ_IX(i, j, ld)] = 0.0;
-                // 2x2 corners (rows i-1,i x cols i-1,i) of A and B: row rotation first
-                double a00 = A[GQZ_IX(i - 1, i - 1, ld)], a01 = A[GQZ_IX(i - 1, i, ld)];
-                double a10 = A[GQZ_IX(i, i - 1, ld)], a11 = A[GQZ_IX(i, i, ld)];
-                double b00 = B[GQZ_IX(i - 1, i - 1, ld)], b01 = B[GQZ_IX(i - 1, i, ld)];
-                double b11 = B[GQZ_IX(i, i, ld)];
-                double na00 = c * a00 + s * a10, na10 = c * a10 - s * a00;
-                double na01 = c * a01 + s * a11, na11 = c * a11 - s * a01;
-                double nb00 = c * b00, nb10 = -s * b00;
-                double nb01 = c * b01 + s * b11, nb11 = c * b11 - s * b01;
-                // column rotation (i, i-1) zeroing B(i, i-1)
-                d_lartg(nb11, nb10, cc, ss, r);
-                // x = col i, y = col i-1: x' = cc x + ss y ; y' = cc y - ss x
-                double xa0 = na01, ya0 = na00, xa1 = na11, ya1 = na10;
-                A[GQZ_IX(i - 1, i, ld)] = cc * xa0 + ss * ya0; A[GQZ_IX(i - 1, i - 1, ld)] = cc * ya0 - ss * xa0;
-                A[GQZ_IX(i, i, ld)] = cc * xa1 + ss * ya1; A[GQZ_IX(i, i - 1, ld)] = cc * ya1 - ss * xa1;
-                B[GQZ_IX(i - 1, i, ld)] = cc * nb01 + ss * nb00; B[GQZ_IX(i - 1, i - 1, ld)] = cc * nb00 - ss * nb01;
-                B[GQZ_IX(i, i, ld)] = r; B[GQZ_IX(i, i - 1, ld)] = 0.0;
-                sh[0] = c; sh[1] = s; sh[2] = cc; sh[3] = ss;
-            }
-            __syncthreads();
-            const double c = sh[0], s = sh[1], cc = sh[2], ss = sh[3];
-            // row rotation: A cols [j+1, n) \ {i-1, i}; B cols [i+1, n); Q cols (i-1, i)
-            // column rotation: A rows [0, ihi] \ {i-1, i}; B rows [0, i-1); Z cols (i, i-1)
-            const int na = n - (j + 1), nb = n - (i + 1), nc = ihi + 1, nd = i - 1;
-            const int nqq = Q ? nq : 0, nzz = Z ? nq : 0;
-            const int tot = na + nb + nqq + nc + nd + nzz;
-            for (int idx = tid; idx < tot; idx += blockDim.x) {
-                int t = idx;
-                if (t < na) {
-                    int col = j + 1 + t;
-                    if (col == i - 1 || col == i) continue;
-                    double x = A[GQZ_IX(i - 1, col, ld)], y = A[GQZ_IX(i, col, ld)];
-                    A[GQZ_IX(i - 1, col, ld)] = c * x + s * y; A[GQZ_IX(i, col, ld)] = c * y - s * x;
-                    continue;
-                }
-                t -= na;
-                if (t < nb) {
-                    int col = i + 1 + t;
-                    double x = B[GQZ_IX(i - 1, col, ld)], y = B[GQZ_IX(i, col, ld)];
-                    B[GQZ_IX(i - 1, col, ld)] = c * x + s * y; B[GQZ_IX(i, col, ld)] = c * y - s * x;
-                    continue;
-                }
-                t -= nb;
-                if (t < nqq) {
-                    double x = Q[GQZ_IX(t, i - 1, ldq)], y = Q[GQZ_IX(t, i, ldq)];
-                    Q[GQZ_IX(t, i - 1, ldq)] = c * x + s * y; Q[GQZ_IX(t, i, ldq)] = c * y - s * x;
-                    continue;
-                }
-                t -= nqq;
-                if (t < nc) {
-                    if (t == i - 1 || t == i) continue;
-                    double x = A[GQZ_IX(t, i, ld)], y = A[GQZ_IX(t, i - 1, ld)];
-                    A[GQZ_IX(t, i, ld)] = cc * x + ss * y; A[GQZ_IX(t, i - 1, ld)] = cc * y - ss * x;
-                    continue;
-                }
-                t -= nc;
-                if (t < nd) {
-                    double x = B[GQZ_IX(t, i, ld)], y = B[GQZ_IX(t, i - 1, ld)];
-                    B[GQZ_IX(t, i, ld)] = cc * x + ss * y; B[GQZ_IX(t, i - 1, ld)] = cc * y - ss * x;
-                    continue;
-                }
-                t -= nd;
-                {
-                    double x = Z[GQZ_IX(t, i, ldq)], y = Z[GQZ_IX(t, i - 1, ldq)];
-                    Z[GQZ_IX(t, i, ldq)] = cc * x + ss * y; Z[GQZ_IX(t, i - 1, ldq)] = cc * y - ss * x;
-                }
-            }
-            __syncthreads();
-        }
-    }
+    wg_gghrd(A, B, ld, n, ilo, ihi, Q, Z, ldq, nq, sh);
 }

@@ -195,7 +195,9 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         return 0;
     }
     const double *sptr = (kwtop > ktop) ? v.H + (size_t)kwtop + (size_t)(kwtop - 1) * v.ld : nullptr;
-    gqz_launch_aed_post(L.st, Hw, Tw, U, V, jw, sptr, W.smlnum, W.ar, W.ai, W.be, scr, L.d_out + 16);
+    double *pscr = L.ws.alloc((size_t)(jw / 2 + 1) * 65);
+    gqz_launch_aed_reorder(L.st, Hw, Tw, U, V, jw, sptr, W.smlnum, pscr, L.d_out + 16);
+    gqz_launch_aed_spike(L.st, Hw, Tw, U, V, jw, sptr, W.ar, W.ai, W.be, scr, L.d_out + 16);
     HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 4 * sizeof(int), hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipStreamSynchronize(L.st));
@@ -435,7 +437,7 @@ size_t arena_bytes(int n, const gqz_params *prm) {
     int nw = piparmq_nw(std::max(n, 1));
     if (prm && prm->nw > nw) nw = prm->nw;
     nw = std::max(std::min(std::max(nw, 1024), n), 64);
-    size_t d = 6 * (size_t)nw * nw + 4 * (size_t)1024 * 1024;
+    size_t d = 6 * (size_t)nw * nw + 80 * (size_t)nw + 4 * (size_t)1024 * 1024;
     return d * sizeof(double);
 }
 
@@ -462,6 +464,10 @@ int decode(char c, const char *opts) {
 }
 
 }  // namespace
+
+int gqz_init_current() { return check_init(); }
+void gqz_set_error(const char *msg) { L.err = msg; }
+
 
 // ================================================================================================
 // C ABI
@@ -531,13 +537,13 @@ int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi, dou
     if (icq < 0) return -2;
     if (icz < 0) return -3;
     if (n < 0) return -4;
-    if (ilo < 1 || ilo > std::max(1, n)) return -5;
-    if (ihi < std::min(ilo, n) || ihi > n) return -6;
+    if (ilo < 1) return -5;
+    if (ihi > n || ihi < ilo - 1) return -6;
     if (ldh < std::max(1, n) || ldt != ldh) return -8;
     if (icq > 0 && (ldq < std::max(1, n) || !Q)) return -15;
     if (icz > 0 && (ldz < std::max(1, n) || !Z)) return -17;
     if (icq > 0 && icz > 0 && ldq != ldz) return -17;
-    if (n == 0) return 0;
+    if (n == 0 || ihi < ilo) return 0;
     try {
         if (check_init()) return -100;
         double t0 = now();
@@ -783,6 +789,42 @@ int gqz_gghrd(char compq, char compz, int n, int ilo, int ihi, double *A, int ld
             HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
             HIPCHK(hipMemcpy2DAsync(B, ldb * sizeof(double), dB, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
             if (icq) HIPCHK(hipMemcpy2DAsync(Q, ldq * sizeof(double), dQ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+        }
+        HIPCHK(hipStreamSynchronize(L.st));
+        HIPCHK(hipFree(d));
+        return info;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+// PDGGHRD with the routine's own Q convention (host pointers): COMPQ='I' returns Q_ht^T,
+// COMPQ='V' returns Q_ht^T * Q1 (see qz_desc.cpp); the product runs on the device.
+int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, int lda, double *B, int ldb,
+                   double *Q, int ldq, double *Z, int ldz) {
+    int icq = decode(compq, "NIV"), icz = decode(compz, "NIV");
+    if (icq < 0) return -1;
+    if (icz < 0) return -2;
+    if (n <= 0) return n < 0 ? -3 : 0;
+    try {
+        if (check_init()) return -100;
+        size_t nn = (size_t)n * n;
+        double *d = nullptr;
+        HIPCHK(hipMalloc((void **)&d, sizeof(double) * 6 * nn));
+        double *dA = d, *dB = d + nn, *dQh = d + 2 * nn, *dZ = d + 3 * nn, *dQ1 = d + 4 * nn, *dW = d + 5 * nn;
+        HIPCHK(hipMemcpy2DAsync(dA, n * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpy2DAsync(dB, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icz == 2) HIPCHK(hipMemcpy2DAsync(dZ, n * sizeof(double), Z, ldz * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        if (icq == 2) HIPCHK(hipMemcpy2DAsync(dQ1, n * sizeof(double), Q, ldq * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
+        else if (icq == 1) gqz_launch_identity(L.st, dQ1, n, n, n);
+        int info = gqz_gghrd_dev(icq ? 'I' : 'N', compz, n, ilo, ihi, dA, n, dB, n, dQh, n, dZ, n);
+        if (info == 0) {
+            if (icq) gqz_gemm(L.st, 1, 0, n, n, n, dQh, n, dQ1, n, dW, n);
+            HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpy2DAsync(B, ldb * sizeof(double), dB, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
+            if (icq) HIPCHK(hipMemcpy2DAsync(Q, ldq * sizeof(double), dW, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
             if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
         }
         HIPCHK(hipStreamSynchronize(L.st));

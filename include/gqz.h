@@ -97,6 +97,15 @@ int gqz_pdgghrd_host(char compq, char compz, int n, int ilo, int ihi,
                      double *Q, const int *descq, double *Z, const int *descz,
                      double *work, int lwork);
 
+/* PDGGHRD with the routine's implemented Q convention (rotations applied to Q from the left;
+ * the drivers transpose Q afterwards, testing/EXRAND2.f:346-356): COMPQ='I' returns Q_ht^T,
+ * COMPQ='V' returns Q_ht^T * Q1 where A_out = Q_ht^T A_in Z.  Host pointers; runs on the device. */
+int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, int lda,
+                   double *B, int ldb, double *Q, int ldq, double *Z, int ldz);
+/* PIPARMQ tuning values (src/pdhgeqz/piparmq.f:300-531) for ISPEC 50..55 and active size nh;
+ * the Fortran shim's PILAENVX answers from it. */
+int gqz_piparmq(int ispec, int nh);
+
 /* ---- residual check (PCOMPRES, tools/PQZHLPS.f:19-319) on device data ------------------------
  * out[0] = ||S - Q^T A Z||_F/||A||_F, out[1] likewise for B, out[2] = ||I - Q^T Q||_F/(n eps),
  * out[3] likewise for Z. */
@@ -126,6 +135,14 @@ int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,
 int gqz_batched_dev(int count, const int *n, const long *off, const long *eoff,
                     double *A, double *B, double *Q, double *Z,
                     double *alphar, double *alphai, double *beta, int *info, int wg_threads);
+/* n, off, eoff are host arrays; A, B, Q, Z, alphar, alphai, beta, info are device arrays.
+ * wg_threads: 0 = tuned shape (or the size heuristic before tuning), else 128/256/512/1024.
+ * Repeated calls with the same arguments replay a captured hipGraph. */
+
+/* Autotuner for the batched launch: times every (workgroup size, groups per CU) candidate on a
+ * synthetic sample drawn at the quantiles of the size list n (host array) and keeps the fastest
+ * for later gqz_batched_dev calls with wg_threads = 0. */
+int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,135 @@
+"""CPU tests of the drop-in boundary: the C ABI library and the Fortran shim load and export every
+entry point declared in include/gqz.h; argument checking, INFO codes and the workspace-query
+protocol follow PDHGEQZ.f:384-444 and PDGGHRD.f:300-336; without a GPU the library refuses to
+run (there is no CPU fallback).  No compute call reaches a GPU here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "gevp-pdhgeqz_amd")
+HDR = os.path.join(ROOT, "include", "gqz.h")
+LIB = os.path.join(PKG, "libgqz.so")
+FLIB = os.path.join(PKG, "libpdhgeqz_f.so")
+
+
+def declared():
+    txt = open(HDR).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:int|void|const char\s*\*|void\s*\*)\s*\*?\s*(gqz_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip("libgqz.so not built")
+    return C.CDLL(LIB)
+
+
+def test_header_declares_core_entry_points():
+    names = declared()
+    for f in ("gqz_hgeqz_dev", "gqz_gghrd_dev", "gqz_pdhgeqz_host", "gqz_pdgghrd_host", "gqz_compres_dev",
+              "gqz_init", "gqz_batched_dev"):
+        assert f in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [f for f in declared() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_fortran_shim_exports():
+    if not os.path.exists(FLIB):
+        pytest.skip("Fortran shim not built (amdflang missing)")
+    F = C.CDLL(FLIB)
+    for sym in ("pdhgeqz_", "pdgghrd_", "kkqzconf_", "pilaenvx_"):
+        assert hasattr(F, sym)
+
+
+def _desc(n, nb=64, mb=None, lld=None):
+    mb = nb if mb is None else mb
+    return (C.c_int * 9)(1, 0, n, n, mb, nb, 0, 0, n if lld is None else lld)
+
+
+def _host_call(lib, job=b"S", cq=b"I", cz=b"I", n=8, ilo=1, ihi=None, nb=64, mb=None, ldwork=None, liwork=None):
+    ihi = n if ihi is None else ihi
+    m = max(n, 1)
+    H = np.zeros((m, m), order="F"); T = np.eye(m, order="F")
+    Q = np.eye(m, order="F"); Z = np.eye(m, order="F")
+    ar = np.zeros(m); ai = np.zeros(m); be = np.zeros(m)
+    ldwork = max(1, n * nb) if ldwork is None else ldwork
+    liwork = max(1, 2 * n) if liwork is None else liwork
+    dw = np.zeros(max(1, ldwork)); iw = np.zeros(max(1, liwork), dtype=np.int32)
+    d = _desc(max(n, 1), nb, mb)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.gqz_pdhgeqz_host.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_int]
+    info = lib.gqz_pdhgeqz_host(job, cq, cz, n, ilo, ihi, vp(H), d, vp(T), d, vp(ar), vp(ai), vp(be),
+                                vp(Q), d, vp(Z), d, vp(dw), ldwork, vp(iw), liwork)
+    return info, dw, iw
+
+
+@pytest.mark.parametrize("kw,expect", [
+    (dict(job=b"X"), -1), (dict(cq=b"Q"), -2), (dict(cz=b"?"), -3), (dict(n=-1), -4),
+    (dict(ilo=0), -5), (dict(ihi=9), -6), (dict(nb=4), -7), (dict(mb=32), -8),
+    (dict(ldwork=10), -9), (dict(liwork=3), -10)])
+def test_pdhgeqz_info_codes(lib, kw, expect):
+    """INFO codes of PDHGEQZ.f:384-405 and :435-444 (no GPU needed: checks come first)."""
+    info, _, _ = _host_call(lib, **kw)
+    assert info == expect
+
+
+def test_pdhgeqz_workspace_query(lib):
+    """LDWORK = -1 / LIWORK = -1 returns the optimum in DWORK(1), IWORK(1) (PDHGEQZ.f:409-430)."""
+    info, dw, iw = _host_call(lib, n=50, nb=60, ldwork=-1, liwork=-1)
+    assert info == 0 and dw[0] == 50 * 60 and iw[0] == 100
+
+
+def test_pilaenvx_via_c(lib):
+    lib.gqz_piparmq.argtypes = [C.c_int, C.c_int]
+    assert lib.gqz_piparmq(50, 1000) == 6000
+    assert (lib.gqz_piparmq(53, 8000), lib.gqz_piparmq(51, 8000), lib.gqz_piparmq(52, 8000)) == (256, 384, 7)
+    assert (lib.gqz_piparmq(53, 16000), lib.gqz_piparmq(51, 16000), lib.gqz_piparmq(52, 16000)) == (512, 768, 5)
+
+
+def test_fortran_pilaenvx_and_info():
+    """The Fortran entry points pass arguments by reference with hidden character lengths."""
+    if not os.path.exists(FLIB):
+        pytest.skip("Fortran shim not built")
+    F = C.CDLL(FLIB)
+    ic = lambda v: C.byref(C.c_int(v))
+    F.pilaenvx_.restype = C.c_int
+    name = C.c_char_p(b"PDHGEQZ0"); opts = C.c_char_p(b"")
+    v = F.pilaenvx_(ic(0), ic(53), name, opts, ic(16000), ic(1), ic(16000), ic(0), C.c_size_t(8), C.c_size_t(0))
+    assert v == 512
+    # a bad JOB returns INFO = -1 through the Fortran ABI
+    n = 4
+    d = _desc(n)
+    H = np.zeros((n, n), order="F"); T = np.eye(n, order="F")
+    e = np.zeros(n); dw = np.zeros(n * 64); iw = np.zeros(2 * n, dtype=np.int32)
+    info = C.c_int(123)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    F.pdhgeqz_(C.c_char_p(b"X"), C.c_char_p(b"I"), C.c_char_p(b"I"), ic(n), ic(1), ic(n), vp(H), d, vp(T), d,
+               vp(e), vp(e), vp(e), vp(H), d, vp(H), d, vp(dw), ic(n * 64), vp(iw), ic(2 * n), C.byref(info),
+               C.c_size_t(1), C.c_size_t(1), C.c_size_t(1))
+    assert info.value == -1
+
+
+def test_no_gpu_fails_loudly(lib):
+    """Without a GPU the library reports an error instead of computing on the CPU."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except Exception:
+        pass
+    info, _, _ = _host_call(lib, n=8)
+    assert info == -100
+    lib.gqz_last_error.restype = C.c_char_p
+    assert b"GPU" in lib.gqz_last_error() or b"HIP" in lib.gqz_last_error() or b"hip" in lib.gqz_last_error()

@@ -1,0 +1,224 @@
+"""CPU tests: pin the oracle (oracle/qz_oracle.c) against LAPACK, closed forms, constructed
+pencils with known spectra and the committed golden fixtures.  No GPU needed."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle_ffi as O
+import qzcheck as Q
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz")
+
+
+def test_generator_fixture():
+    """SplitMix64 + Box-Muller generator: first values and checksums match the fixture."""
+    g = np.load(GOLD)
+    np.testing.assert_array_equal(O.normal(1, (16,)), g["normal_seed1_16"])
+    np.testing.assert_array_equal(O.uniform(7, (16,)), g["uniform_seed7_16"])
+    x = O.normal(12345, (100000,))
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1.0) < 0.01
+
+
+def test_piparmq_table():
+    """PIPARMQ defaults (piparmq.f:445-517) at the config sizes: NS/NW/NIBBLE = 256/384/7 at
+    n=8000, 512/768/6 at 12000, 512/768/5 at 16000 (UserGuide p.13)."""
+    L = O.lib()
+    assert (L.orc_piparmq_ns(8000), L.orc_piparmq_nw(8000), L.orc_piparmq_nibble(8000)) == (256, 384, 7)
+    assert (L.orc_piparmq_ns(12000), L.orc_piparmq_nw(12000), L.orc_piparmq_nibble(12000)) == (512, 768, 6)
+    assert (L.orc_piparmq_ns(16000), L.orc_piparmq_nw(16000), L.orc_piparmq_nibble(16000)) == (512, 768, 5)
+    assert L.orc_piparmq_ns(100) == 10 and L.orc_piparmq_nw(100) == 10
+    assert L.orc_piparmq_nw(1000) == 96 and L.orc_piparmq_nibble(1000) == 14
+
+
+def test_lartg_larfg():
+    import ctypes as C
+    L = O.lib()
+    c, s, r = C.c_double(), C.c_double(), C.c_double()
+    for f, g in [(3.0, 4.0), (-1.0, 2.0), (0.0, -5.0), (7.0, 0.0)]:
+        L.orc_lartg(f, g, C.byref(c), C.byref(s), C.byref(r))
+        assert abs(c.value * f + s.value * g - r.value) < 1e-15 * abs(r.value) + 1e-300
+        assert abs(-s.value * f + c.value * g) < 1e-15 * max(abs(f), abs(g))
+        assert abs(c.value ** 2 + s.value ** 2 - 1.0) < 1e-15
+    x = np.array([2.0, -1.0, 0.5])
+    alpha = C.c_double(x[0]); tail = x[1:].copy(); tau = C.c_double()
+    L.orc_larfg(3, C.byref(alpha), O.ptr(tail), 1, C.byref(tau))
+    v = np.r_[1.0, tail]
+    Hm = np.eye(3) - tau.value * np.outer(v, v)
+    np.testing.assert_allclose(Hm @ x, [alpha.value, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(Hm @ Hm.T, np.eye(3), atol=1e-15)
+
+
+def test_shift_column_closed_form():
+    """QZFCOL role: v is parallel to p(H T^-1) e1 with p(z) = (z - l1)(z - l2) (closed form)."""
+    rng = np.random.default_rng(3)
+    for cplx in (False, True):
+        H = np.triu(rng.standard_normal((4, 4)), -1)
+        T = np.triu(rng.standard_normal((4, 4))) + 3 * np.eye(4)
+        if cplx:
+            l1, l2 = 0.3 + 0.7j, 0.3 - 0.7j
+            args = (0.3, 0.7, 1.0, 0.3, -0.7, 1.0)
+        else:
+            l1, l2 = 0.4, -1.1
+            args = (0.4, 0.0, 1.0, -1.1, 0.0, 1.0)
+        M = H @ np.linalg.inv(T)
+        p = (M - l1 * np.eye(4)) @ (M - l2 * np.eye(4)) @ np.eye(4)[:, 0]
+        p = np.real(p)
+        v = np.zeros(3)
+        O.lib().orc_shift_col(O.ptr(O.fmat(H)), 4, O.ptr(O.fmat(T)), 4, *args, O.ptr(v))
+        assert abs(p[3]) < 1e-12
+        cosang = abs(v @ p[:3]) / (np.linalg.norm(v) * np.linalg.norm(p[:3]))
+        assert abs(cosang - 1.0) < 1e-13
+
+
+def test_invhse_annihilates():
+    """INVHSE: T (I - tau v v^T) has zeros below the diagonal in its first column."""
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        T = rng.standard_normal((3, 3))
+        v = np.zeros(3); tau = np.zeros(1)
+        O.lib().orc_invhse3(O.ptr(O.fmat(T)), 3, O.ptr(v), O.ptr(tau))
+        P = np.eye(3) - tau[0] * np.outer(v, v)
+        R = T @ P
+        assert np.abs(R[1:, 0]).max() < 1e-14 * np.abs(T).max()
+        np.testing.assert_allclose(P @ P.T, np.eye(3), atol=1e-15)
+
+
+def test_lagv2_2x2_forms():
+    """DLAGV2 role: real pair -> triangular; complex pair -> B diagonal positive; same eigenvalues."""
+    rng = np.random.default_rng(9)
+    for trial in range(40):
+        A = rng.standard_normal((2, 2))
+        B = np.triu(rng.standard_normal((2, 2))) + np.diag([0.5, 0.5]) * np.sign(rng.standard_normal(2))
+        A0, B0 = A.copy(), B.copy()
+        Af, Bf = O.fmat(A), O.fmat(B)
+        ar, ai, be = np.zeros(2), np.zeros(2), np.zeros(2)
+        QL, ZR = np.zeros(4), np.zeros(4)
+        O.lib().orc_lagv2(O.ptr(Af), 2, O.ptr(Bf), 2, O.ptr(ar), O.ptr(ai), O.ptr(be), O.ptr(QL), O.ptr(ZR))
+        QLm, ZRm = QL.reshape(2, 2, order="F"), ZR.reshape(2, 2, order="F")
+        np.testing.assert_allclose(QLm.T @ A0 @ ZRm, Af, atol=1e-13 * (1 + np.abs(A0).max()))
+        np.testing.assert_allclose(QLm.T @ B0 @ ZRm, Bf, atol=1e-13 * (1 + np.abs(B0).max()))
+        ref = sl.eigvals(A0, B0)
+        got = (ar + 1j * ai) / be
+        d = min(np.abs(got - ref).max(), np.abs(got[::-1] - ref).max())
+        assert d < 1e-10 * (1 + np.abs(ref).max())
+        if ai[0] == 0:
+            assert Af[1, 0] == 0.0 and Bf[1, 0] == 0.0
+        else:
+            assert Bf[0, 1] == 0.0 and Bf[0, 0] > 0 and Bf[1, 1] > 0
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (1, 2), (2, 1), (2, 2)])
+def test_tgex2_swaps_blocks(n1, n2):
+    """DTGEX2 role: swapping two blocks keeps the pencil equivalent and exchanges eigenvalues."""
+    rng = np.random.default_rng(10 * n1 + n2)
+    m = n1 + n2
+
+    def block(k):
+        if k == 1:
+            return np.array([[rng.uniform(1, 2)]]), np.array([[rng.uniform(0.5, 1)]])
+        a, b = rng.uniform(0.5, 1.5), rng.uniform(1, 2)
+        return np.array([[a, b], [-b, a]]), np.diag(rng.uniform(0.5, 1.5, 2))
+
+    S11, T11 = block(n1); S22, T22 = block(n2)
+    S = np.zeros((m, m)); T = np.zeros((m, m))
+    S[:n1, :n1], T[:n1, :n1] = S11, T11
+    S[n1:, n1:], T[n1:, n1:] = S22, T22
+    S[:n1, n1:] = rng.standard_normal((n1, n2)); T[:n1, n1:] = rng.standard_normal((n1, n2))
+    S0, T0 = S.copy(), T.copy()
+    Sf, Tf, Qf, Zf = O.fmat(S), O.fmat(T), O.fmat(np.eye(m)), O.fmat(np.eye(m))
+    r = O.lib().orc_tgex2(m, O.ptr(Sf), m, O.ptr(Tf), m, O.ptr(Qf), m, O.ptr(Zf), m, 0, n1, n2)
+    assert r == 0
+    np.testing.assert_allclose(Qf.T @ S0 @ Zf, Sf, atol=1e-13)
+    np.testing.assert_allclose(Qf.T @ T0 @ Zf, Tf, atol=1e-13)
+    assert np.abs(Sf[n2:, :n2]).max() == 0.0 and np.abs(np.tril(Tf, -1)).max() == 0.0
+    top_ref = sl.eigvals(S22, T22)
+    top_got = sl.eigvals(Sf[:n2, :n2], Tf[:n2, :n2])
+    d = min(np.abs(top_got - top_ref).max(), np.abs(top_got[::-1] - top_ref).max())
+    assert d < 1e-12 * (1 + np.abs(top_ref).max())
+
+
+def test_ht_reduction_invariants():
+    """QR(B) + PDGGHRD: H upper Hessenberg, T upper triangular, backward error and
+    orthogonality at PCOMPRES thresholds (EXRAND2.f:404-407)."""
+    A, B = Q.pencil_randn(60, 1)
+    H, T, Qm, Zm = O.ht_reduce(A, B)
+    assert np.abs(np.tril(H, -2)).max() == 0.0 and np.abs(np.tril(T, -1)).max() == 0.0
+    ra, rb, rq, rz = Q.compres(A, B, H, T, Qm, Zm)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 40, 120, 260])
+def test_oracle_qz_vs_lapack(n):
+    """Oracle multishift QZ with AED: PCOMPRES thresholds, Schur structure, and the eigenvalue set
+    equal to LAPACK DGGES's within the condition-weighted chordal bound."""
+    A, B = Q.pencil_randn(n, 100 + n)
+    H, T, Qm, Zm = O.ht_reduce(A, B)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=2, compz=2, Q=Qm, Z=Zm)
+    assert info == 0
+    ra, rb, rq, rz = Q.compres(A, B, S, P, Q2, Z2)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+    below, tb, ok2 = Q.schur_structure(S, P)
+    assert below == 0.0 and tb == 0.0 and ok2
+    assert (be >= 0).all()
+    ratio, _, _ = Q.match_eigs(ar, ai, be, A, B)
+    assert ratio < 1.0
+
+
+def test_oracle_n1_n2_closed_forms():
+    """n = 1: (alpha, beta) = (a, |b|) with sign; n = 2 eigenvalues from the quadratic."""
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(np.array([[3.0]]), np.array([[-2.0]]))
+    assert info == 0 and be[0] == 2.0 and ar[0] == -3.0
+    A = np.array([[1.0, 2.0], [3.0, 4.0]]); B = np.array([[2.0, 1.0], [0.0, 1.0]])
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(A, B)
+    # det(A - l B) = (1 - 2l)(4 - l) - (2 - l) 3 = 2 l^2 - 6 l - 2
+    roots = np.sort(np.roots([2.0, -6.0, -2.0]))
+    np.testing.assert_allclose(np.sort(ar / be), roots, rtol=1e-13)
+
+
+@pytest.mark.parametrize("n,k", [(40, 4), (150, 12), (90, 45)])
+def test_oracle_known_infinite_count(n, k):
+    """Constructed pencil with exactly k semisimple infinite eigenvalues: beta = 0 count == k and
+    the finite eigenvalues equal the constructed ones."""
+    A, B, lam, kinf = Q.pencil_known(n, 7 + n, n_inf=k)
+    H, T, Qm, Zm = O.ht_reduce(A, B)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=2, compz=2, Q=Qm, Z=Zm)
+    assert info == 0
+    assert int((be == 0).sum()) == kinf
+    fin = be != 0
+    got = np.sort((ar[fin] + 1j * ai[fin]) / be[fin])
+    np.testing.assert_allclose(np.sort_complex(got), np.sort_complex(lam.astype(complex)), rtol=1e-8, atol=1e-8)
+
+
+def test_oracle_vs_golden_lapack():
+    """The committed golden fixture (LAPACK DGGES eigenvalues + residual bounds for seeded
+    pencils, written by tests/golden/make_golden.py) is reproduced by the oracle."""
+    g = np.load(GOLD)
+    for n in (16, 48):
+        A, B = Q.pencil_randn(n, int(g["seed_%d" % n]))
+        np.testing.assert_array_equal(A[:2, :2], g["A_%d_corner" % n])
+        H, T, Qm, Zm = O.ht_reduce(A, B)
+        S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=2, compz=2, Q=Qm, Z=Zm)
+        a2, b2 = g["lapack_alpha_%d" % n], g["lapack_beta_%d" % n]
+        D = Q.chordal((ar + 1j * ai)[:, None], be[:, None], a2[None, :], b2[None, :])
+        from scipy.optimize import linear_sum_assignment
+        ri, ci = linear_sum_assignment(D)
+        assert D[ri, ci].max() < 1e-10
+
+
+def test_oracle_chase_window_equivalence():
+    """PDHGEQZB chase of a chain equals successive applications (invariants: orthogonal U, V,
+    U^T H0 V = H1, Hessenberg-with-bulges structure after push-off)."""
+    n = 40
+    H0, T0 = Q.pencil_ht_random(n, 2)
+    # build a chain of 3 bulges with the oracle's own intro via one double-shift sweep each
+    H, T = O.fmat(H0), O.fmat(T0)
+    U = O.fmat(np.eye(n)); V = O.fmat(np.eye(n))
+    sr = np.array([0.5, -0.3, 1.2, 0.7, -1.0, 0.2]); si = np.zeros(6); sb = np.ones(6)
+    # chain already at tails 0,3,6 is not needed for push-off invariants: use movedist = n
+    O.lib().orc_chase_window(n, O.ptr(H), n, O.ptr(T), n, O.ptr(U), O.ptr(V), n, 1, n, 1,
+                             O.ptr(sr), O.ptr(si), O.ptr(sb))
+    np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-13)
+    np.testing.assert_allclose(U.T @ H0 @ V, H, atol=1e-12 * np.abs(H0).max())
+    np.testing.assert_allclose(U.T @ T0 @ V, T, atol=1e-12 * np.abs(T0).max())
