@@ -1,0 +1,194 @@
+"""Benchmark of the multishift QZ hot path on one MI355X.
+
+Workload (default): Schur form (JOB='S', COMPQ=COMPZ='I') of an n=16000 Hessenberg-triangular pencil
+drawn as in testing/EXRAND1.f:243-279 (N(0,1) upper parts, chi-distributed subdiagonal of H and
+diagonal of T: the distribution of an HT-reduced N(0,1) pencil), with the PIPARMQ default parameters
+(NS=512, NW=768, NIBBLE=5 at n=16000) or the learned parameter policy (--params learned).
+A "step" is one complete QZ of that pencil; the pencil is generated on the device and restored from a
+pristine device copy before each step (inputs resident in HBM when timing starts).
+
+Output: one JSON line with the QZ time in seconds, the roofline of the dominant kernel class
+(HIP-event time on the library stream, algorithmic bytes or flops of its launches) and a CPU baseline
+(the oracle's QZ on a bounded sample, cubically extrapolated to the workload's n).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "gevp-pdhgeqz_amd"), os.path.join(ROOT, "tests"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_HBM = 8.0e12          # B/s, MI355X HBM3E
+PEAK_FP64 = 78.6e12        # flop/s, MI355X FP64 matrix (dense)
+RIDGE = PEAK_FP64 / PEAK_HBM
+
+
+class KTime(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("launches", C.c_long), ("ms", C.c_double), ("flops", C.c_double),
+                ("bytes", C.c_double)]
+
+
+def ht_random_device(n, seed, torch, gqz):
+    """EXRAND1-style HT pencil generated on the device (column-major)."""
+    import numpy as np
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    H = gqz.colmajor(n); T = gqz.colmajor(n)
+    # column-major n x n == transpose of a row-major tensor: fill the row-major storage with the
+    # lower triangle of its transpose
+    Hs = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g)
+    H.copy_(Hs.t()); del Hs
+    H.triu_()
+    Ts = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g)
+    T.copy_(Ts.t()); del Ts
+    T.triu_()
+    rng = np.random.default_rng(seed)
+    sub = np.sqrt(rng.chisquare(np.arange(n - 1, 0, -1)))
+    dia = np.sqrt(rng.chisquare(np.arange(n, 0, -1)))
+    idx = torch.arange(n - 1, device="cuda")
+    H[idx + 1, idx] = torch.from_numpy(sub).cuda()
+    ii = torch.arange(n, device="cuda")
+    T[ii, ii] = torch.from_numpy(dia).cuda()
+    return H, T
+
+
+def cpu_baseline(n_target, n_sample, seed):
+    """Oracle QZ (the C restatement, one core) on an n_sample HT-random pencil; cubic extrapolation."""
+    import numpy as np
+    import oracle_ffi as O
+    import qzcheck as QC
+    H, T = QC.pencil_ht_random(n_sample, seed)
+    t0 = time.perf_counter()
+    _, _, _, _, _, _, _, info, _ = O.hgeqz(H, T)
+    dt = time.perf_counter() - t0
+    return {"value": dt * (n_target / n_sample) ** 3, "unit": "s", "cores": 1, "kind": "port",
+            "sample": "oracle QZ (JOB=S, COMPQ=COMPZ=I) of one n=%d HT-random pencil: %.2f s measured on 1 core, "
+                      "scaled by (n/%d)^3 to n=%d; info=%d" % (n_sample, dt, n_sample, n_target, info)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--params", choices=["default", "learned"], default="default")
+    ap.add_argument("--warmup-n", type=int, default=2000, help="pencil order of the warmup steps")
+    ap.add_argument("--cpu-sample-n", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--ktime", type=int, default=1, help="record per-class kernel event times")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this QZ build has no multi-GPU path (see the sharding "
+              "design); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gqz
+    L = gqz.init(0)
+    L.gqz_ktime_enable.argtypes = [C.c_uint]
+    L.gqz_ktime_get.argtypes = [C.POINTER(KTime), C.c_int]
+    n = args.n
+    prm = gqz.GqzParams()
+    policy = None
+    if args.params == "learned":
+        import gqz_policy
+        policy = gqz_policy.install_learned(L)
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    # warmup steps (same pipeline, smaller pencil: loads code objects, sizes the workspace)
+    for w in range(args.warmup):
+        Hw, Tw = ht_random_device(args.warmup_n, 1000 + w, torch, gqz)
+        Qw = gqz.colmajor(args.warmup_n); Zw = gqz.colmajor(args.warmup_n)
+        _, _, _, info, _ = gqz.pdhgeqz("S", "I", "I", Hw, Tw, Qw, Zw, params=prm)
+        log("warmup %d: n=%d info=%d" % (w, args.warmup_n, info))
+        del Hw, Tw, Qw, Zw
+    torch.cuda.synchronize()
+
+    H0, T0 = ht_random_device(n, args.seed, torch, gqz)
+    H = gqz.colmajor(n); T = gqz.colmajor(n); Q = gqz.colmajor(n); Z = gqz.colmajor(n)
+    torch.cuda.synchronize()
+    L.gqz_ktime_enable((1 << 7) - 1 if args.ktime else 0)
+    times, stats, infos = [], [], []
+    torch.cuda.synchronize()
+    t_all0 = time.perf_counter()
+    for s in range(args.steps):
+        H.copy_(H0); T.copy_(T0)
+        t0 = time.perf_counter()
+        ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        stats.append(st); infos.append(info)
+        log("step %d: n=%d %.3f s info=%d sweeps=%d aed=%d" % (s, n, times[-1], info, st["sweeps"], st["aed_calls"]))
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t_all0
+    kt = (KTime * 8)()
+    nk = L.gqz_ktime_get(kt, 8)
+    L.gqz_ktime_enable(0)
+    classes = [dict(name=kt[i].name.decode(), launches=kt[i].launches, ms=kt[i].ms, flops=kt[i].flops,
+                    bytes=kt[i].bytes) for i in range(max(nk, 0))]
+    ms_per_step = 1000.0 * t_all / args.steps
+    roof = None
+    if classes and any(c["launches"] for c in classes):
+        dom = max(classes, key=lambda c: c["ms"])
+        avg_s = dom["ms"] / 1000.0 / dom["launches"]
+        if dom["flops"] > 0 and dom["flops"] / max(dom["bytes"], 1.0) > RIDGE:
+            ach = dom["flops"] / dom["launches"] / avg_s / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        else:
+            ach = dom["bytes"] / dom["launches"] / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        roof["traffic"] = None
+        roof["kernel"] = dom["name"]
+        roof["avg_launch_us"] = 1e6 * avg_s
+        roof["launches_per_step"] = dom["launches"] / args.steps
+        roof["share_of_step"] = dom["ms"] / 1000.0 / t_all
+    check = None
+    if not args.no_check:
+        res = gqz.compres(H0, T0, H, T, Q, Z)
+        be_h = be.cpu()
+        check = {"resA": res[0], "resB": res[1], "resQ_neps": res[2], "resZ_neps": res[3],
+                 "n_infinite": int((be_h == 0).sum()), "info": infos[-1]}
+        log("check:", check)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, args.cpu_sample_n, args.seed)
+    st = stats[-1]
+    out = {
+        "metric": "QZ seconds, n=%d pencil, %s parameters" % (n, args.params),
+        "value": t_all / args.steps,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: EXRAND1-style random Hessenberg-triangular pencil generated on the device, seed %d"
+                % args.seed,
+        "config": {"workload": "multishift QZ (JOB=S, COMPQ=COMPZ=I) of an n=%d HT pencil" % n, "n": n,
+                   "params": args.params, "ns": st.get("shifts_used", 0), "sweeps": st["sweeps"],
+                   "aed_calls": st["aed_calls"], "iterations": st["iters"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "check": check,
+        "kernel_classes": classes,
+        "step_seconds": times,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,9 +113,64 @@ struct Run {
     int last_ns = 0, last_nw = 0, last_nibble = 0, last_swept = 0, last_ndinf = 0;
 };
 
+// ---- per-class kernel timing: HIP events on the library stream around each launch --------------
+struct KTimer {
+    unsigned mask = 0;
+    struct Pend { int cls; hipEvent_t e0, e1; double flops, bytes; };
+    std::vector<Pend> pend;
+    std::vector<hipEvent_t> pool;
+    long launches[GQZ_KC_COUNT] = {};
+    double ms[GQZ_KC_COUNT] = {}, flops[GQZ_KC_COUNT] = {}, bytes[GQZ_KC_COUNT] = {};
+} KT;
+
+hipEvent_t kt_event() {
+    if (!KT.pool.empty()) { hipEvent_t e = KT.pool.back(); KT.pool.pop_back(); return e; }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+// returns a slot (or -1 when the class is not timed); the matching kt_end records the stop event
+int kt_begin(int cls) {
+    if (!(KT.mask & (1u << cls))) return -1;
+    KTimer::Pend p = { cls, kt_event(), kt_event(), 0.0, 0.0 };
+    HIPCHK(hipEventRecord(p.e0, L.st));
+    KT.pend.push_back(p);
+    return (int)KT.pend.size() - 1;
+}
+
+void kt_end(int slot, double flops, double bytes) {
+    if (slot < 0) return;
+    KTimer::Pend &p = KT.pend[slot];
+    p.flops = flops; p.bytes = bytes;
+    HIPCHK(hipEventRecord(p.e1, L.st));
+}
+
+// fold completed launches into the totals (the stream has been synchronised)
+void kt_drain() {
+    for (auto &p : KT.pend) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, p.e0, p.e1));
+        KT.launches[p.cls]++; KT.ms[p.cls] += t; KT.flops[p.cls] += p.flops; KT.bytes[p.cls] += p.bytes;
+        KT.pool.push_back(p.e0); KT.pool.push_back(p.e1);
+    }
+    KT.pend.clear();
+}
+
 void sync_read(void *dst, const void *src, size_t bytes) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipStreamSynchronize(L.st));
+    if (!KT.pend.empty()) kt_drain();
+}
+
+// algorithmic work of one multi-slab launch: 2 m k^2 flops per slab; each slab read and written
+// once (16 m k bytes) plus its k x k factor (8 k^2 bytes)
+void slab_work(const GqzSlabJobs &J, int k, double &flops, double &bytes) {
+    flops = 0.0; bytes = 0.0;
+    for (int q = 0; q < J.njobs; ++q) {
+        flops += 2.0 * J.m[q] * (double)k * k;
+        bytes += 16.0 * J.m[q] * (double)k + 8.0 * (double)k * k;
+    }
 }
 
 // ---- PDHGEQZ6 role: apply the window's U, V outside the window [p, p+w) -------------------
@@ -126,7 +181,11 @@ void apply_window(Run &R, const View &v, int p, int w, const double *U, const do
         Jl.njobs = 2;
         Jl.C[0] = v.H + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[0] = v.ld; Jl.m[0] = ncol; Jl.F[0] = U; Jl.ldf[0] = w;
         Jl.C[1] = v.T + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[1] = v.ld; Jl.m[1] = ncol; Jl.F[1] = U; Jl.ldf[1] = w;
+        double fl, by;
+        slab_work(Jl, w, fl, by);
+        int ks = kt_begin(GQZ_KC_SLAB);
         gqz_slab_left_multi(L.st, Jl, w);
+        kt_end(ks, fl, by);
         R.st.slab_launches++;
     }
     GqzSlabJobs Jr = {};
@@ -138,7 +197,14 @@ void apply_window(Run &R, const View &v, int p, int w, const double *U, const do
     if (v.Q) { Jr.C[q] = v.Q + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = U; Jr.ldf[q] = w; ++q; }
     if (v.Z) { Jr.C[q] = v.Z + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = V; Jr.ldf[q] = w; ++q; }
     Jr.njobs = q;
-    if (q > 0) { gqz_slab_right_multi(L.st, Jr, w); R.st.slab_launches++; }
+    if (q > 0) {
+        double fl, by;
+        slab_work(Jr, w, fl, by);
+        int ks = kt_begin(GQZ_KC_SLAB);
+        gqz_slab_right_multi(L.st, Jr, w);
+        kt_end(ks, fl, by);
+        R.st.slab_launches++;
+    }
 }
 
 // ---- small active block: LDS-resident base QZ, then the off-block updates ------------------
@@ -152,7 +218,9 @@ int small_block(Run &R, const View &v, int lo, int hi) {
     J.ld = v.ld; J.m = m; J.U = U; J.V = V; J.ldu = m;
     J.ar = v.ar + lo; J.ai = v.ai + lo; J.be = v.be + lo;
     J.info = L.d_out + 8; J.ttol = R.ttol; J.smlnum = v.smlnum;
+    int kk = kt_begin(GQZ_KC_SMALL);
     gqz_launch_small1(L.st, J);
+    kt_end(kk, 0.0, 64.0 * m * m);
     apply_window(R, v, lo, m, U, V);
     sync_read(L.h_out + 8, L.d_out + 8, sizeof(int));
     int info = L.h_out[8];
@@ -183,7 +251,9 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         GqzSmallJob J = {};
         J.H = Hw; J.T = Tw; J.ld = jw; J.m = jw; J.U = U; J.V = V; J.ldu = jw;
         J.ar = W.ar; J.ai = W.ai; J.be = W.be; J.info = L.d_out + 9; J.ttol = R.ttol; J.smlnum = W.smlnum;
+        int kk = kt_begin(GQZ_KC_SMALL);
         gqz_launch_small1(L.st, J);
+        kt_end(kk, 0.0, 64.0 * jw * jw);
         sync_read(L.h_out + 9, L.d_out + 9, sizeof(int));
         info = L.h_out[9];
     } else {
@@ -196,8 +266,12 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     }
     const double *sptr = (kwtop > ktop) ? v.H + (size_t)kwtop + (size_t)(kwtop - 1) * v.ld : nullptr;
     double *pscr = L.ws.alloc((size_t)(jw / 2 + 1) * 65);
+    int kr = kt_begin(GQZ_KC_REORDER);
     gqz_launch_aed_reorder(L.st, Hw, Tw, U, V, jw, sptr, W.smlnum, pscr, L.d_out + 16);
+    kt_end(kr, 0.0, 32.0 * jj);
+    int kp = kt_begin(GQZ_KC_SPIKE);
     gqz_launch_aed_spike(L.st, Hw, Tw, U, V, jw, sptr, W.ar, W.ai, W.be, scr, L.d_out + 16);
+    kt_end(kp, 0.0, 32.0 * jj);
     HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 4 * sizeof(int), hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipStreamSynchronize(L.st));
@@ -205,7 +279,11 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     R.st.swaps += L.h_out[17];
     R.st.swap_fail += L.h_out[18] ? 1 : 0;
     int spike = L.h_out[19];
-    if (kbc > 0 && spike) gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+    if (kbc > 0 && spike) {
+        int kg = kt_begin(GQZ_KC_HT);
+        gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+        kt_end(kg, 0.0, 32.0 * jj);
+    }
     gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? 1 : 0);
     apply_window(R, v, kwtop, jw, U, V);
     sh.assign(3 * (size_t)kbc, 0.0);
@@ -235,7 +313,9 @@ void chase_chain(Run &R, const View &v, int ktop, int kbot, int nb, const double
     int last = (p + w - 1 == kbot);
     a.p = p; a.w = w; a.intro = 1; a.t0 = -1; a.last = last;
     a.steps = last ? w : (w - 3 * nb - 1);
+    int kc = kt_begin(GQZ_KC_CHASE);
     gqz_chase_window(L.st, a);
+    kt_end(kc, 0.0, 48.0 * w * w);
     apply_window(R, v, p, w, U, V);
     R.st.chase_windows++;
     int gtop = p + a.steps;
@@ -245,7 +325,9 @@ void chase_chain(Run &R, const View &v, int ktop, int kbot, int nb, const double
         last = (p + w - 1 == kbot);
         a.p = p; a.w = w; a.intro = 0; a.t0 = 0; a.last = last;
         a.steps = last ? w : (w - 3 * nb - 1);
+        int kc2 = kt_begin(GQZ_KC_CHASE);
         gqz_chase_window(L.st, a);
+        kt_end(kc2, 0.0, 48.0 * w * w);
         apply_window(R, v, p, w, U, V);
         R.st.chase_windows++;
         gtop = p + a.steps;
@@ -370,8 +452,10 @@ int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
         if (it >= maxit) return kbot + 1;
         double ti = now();
         R.st.iters++;
+        int ki = kt_begin(GQZ_KC_INF);
         gqz_launch_inf_defl(L.st, v.H, v.T, v.Q, v.Z, v.ld, v.ldq, v.n, v.nq, ilo, kbot, R.ttol, v.smlnum, v.ar,
                             v.ai, v.be, L.d_out);
+        kt_end(ki, 0.0, 32.0 * (kbot - ilo + 1));
         sync_read(L.h_out, L.d_out, 4 * sizeof(int));
         R.st.t_inf += now() - ti;
         int ktop = L.h_out[0], kt = L.h_out[1], kb = L.h_out[2], ninf = L.h_out[3];
@@ -518,6 +602,31 @@ int gqz_finalize(void) {
 int gqz_sync(void) {
     if (L.dev < 0) return 0;
     return hipStreamSynchronize(L.st) == hipSuccess ? 0 : -1;
+}
+
+int gqz_ktime_enable(unsigned mask) {
+    if (L.dev >= 0) {
+        hipStreamSynchronize(L.st);
+        kt_drain();
+    }
+    KT.mask = mask;
+    for (int c = 0; c < GQZ_KC_COUNT; ++c) { KT.launches[c] = 0; KT.ms[c] = KT.flops[c] = KT.bytes[c] = 0.0; }
+    return 0;
+}
+
+int gqz_ktime_get(gqz_ktime *out, int cap) {
+    static const char *names[GQZ_KC_COUNT] = { "slab_update", "chase_window", "small_qz", "aed_reorder",
+                                               "aed_spike", "ht_givens", "inf_deflation" };
+    if (L.dev >= 0) {
+        if (hipStreamSynchronize(L.st) != hipSuccess) return -1;
+        kt_drain();
+    }
+    int k = std::min(cap, (int)GQZ_KC_COUNT);
+    for (int c = 0; c < k; ++c) {
+        out[c].name = names[c]; out[c].launches = KT.launches[c]; out[c].ms = KT.ms[c];
+        out[c].flops = KT.flops[c]; out[c].bytes = KT.bytes[c];
+    }
+    return k;
 }
 
 int gqz_set_policy(gqz_policy_fn fn, void *user) { L.pol = fn; L.pol_user = user; return 0; }

@@ -84,6 +84,8 @@ def lib():
     L.gqz_chase_window_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.gqz_small_qz_dev.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_double]
+    L.gqz_batched_dev.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.gqz_batched_tune.argtypes = [C.c_int, vp, ip, ip, dp]
     _LIB = L
     return L
 

@@ -128,6 +128,20 @@ int gqz_chase_window_dev(double *H, double *T, int ld, int p, int w, int nb, int
 int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,
                      double *alphar, double *alphai, double *beta, double ttol);
 
+/* ---- kernel timing (no counterpart in the reference) -----------------------------------------
+ * HIP events recorded on the library stream around every launch of the enabled kernel classes
+ * (bit c of mask = class c); per class: launches, summed event time and the algorithmic work of
+ * those launches (flops; bytes = each operand read once and each result written once). */
+enum { GQZ_KC_SLAB = 0, GQZ_KC_CHASE, GQZ_KC_SMALL, GQZ_KC_REORDER, GQZ_KC_SPIKE, GQZ_KC_HT, GQZ_KC_INF,
+       GQZ_KC_COUNT };
+typedef struct {
+    const char *name;
+    long launches;
+    double ms, flops, bytes;
+} gqz_ktime;
+int gqz_ktime_enable(unsigned mask);          /* resets the totals */
+int gqz_ktime_get(gqz_ktime *out, int cap);   /* synchronises; returns the number of classes */
+
 /* ---- batched pencils (no counterpart in the reference) ---------------------------------------
  * count pencils of orders n[i] stored back to back (column-major, ld = n[i]) at offsets off[i]
  * (in doubles) of the device arrays A and B; full pipeline QR(B) + HT + QZ per pencil in one

@@ -1,0 +1,408 @@
+"""GPU parity tests: every kernel family and entry point of the HIP QZ path, called through the C ABI
+(libgqz.so) and the Fortran shim, checked against the oracle on the same seeded inputs, against
+LAPACK (scipy) eigenvalues with condition-weighted chordal tolerances, and against the PCOMPRES
+residuals the reference's drivers gate on (testing/EXRAND1.f:329-332: RESA, RESB < 1e-14 there;
+1e-13 here, the bound the north-star tolerance names; RESQ, RESZ < 10 in units of n eps)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import qzcheck as QC
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-13       # backward error bound (relative Frobenius)
+ORTH_TOL = 10.0       # ||Q^T Q - I||_F / (n eps), EXRAND1.f:329-330
+
+
+def _mods():
+    import torch
+    import gqz
+    import oracle_ffi as O
+    gqz.init(0)
+    return torch, gqz, O
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _note(*a):
+    print(*a, flush=True)
+
+
+# ---- (d) slab updates: FP64 MFMA in-place products -------------------------------------------------
+@pytest.mark.parametrize("m,k", [(16, 4), (300, 64), (257, 200), (130, 520), (5, 7), (1000, 130)])
+def test_slab_update_matches_fp64_product(m, k):
+    torch, gqz, _ = _mods()
+    _note("slab", m, k)
+    rng = np.random.default_rng(m * 1000 + k)
+    Cn = rng.standard_normal((m, k)); F = rng.standard_normal((k, k))
+    Ct = gqz.to_colmajor(torch.from_numpy(Cn)); Ft = gqz.to_colmajor(torch.from_numpy(F))
+    gqz.slab_update("R", Ct, Ft)
+    ref = Cn @ F
+    assert np.abs(_host(Ct) - ref).max() <= 1e-14 * k * np.abs(ref).max()
+    Cn2 = rng.standard_normal((k, m)); Ct2 = gqz.to_colmajor(torch.from_numpy(Cn2))
+    gqz.slab_update("L", Ct2, Ft)
+    ref2 = F.T @ Cn2
+    assert np.abs(_host(Ct2) - ref2).max() <= 1e-14 * k * np.abs(ref2).max()
+
+
+# ---- (b) chain chase window: GPU kernel vs the oracle's DHGEQZ5 restatement --------------------------
+def _chase_setup(n, nb, seed):
+    torch, gqz, O = _mods()
+    A = O.normal(seed, (n, n)); B = O.normal(seed + 1, (n, n))
+    H0, T0, _, _ = O.ht_reduce(A, B)
+    rng = np.random.default_rng(seed)
+    sr = rng.standard_normal(2 * nb); si = np.zeros(2 * nb); sb = np.ones(2 * nb)
+    dev = [torch.tensor(x, dtype=torch.float64, device="cuda") for x in (sr, si, sb)]
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H0)), gqz.to_colmajor(torch.from_numpy(T0))
+    U = gqz.colmajor(n); V = gqz.colmajor(n)
+    gqz.chase_window(Hd, Td, 0, n, nb, -1, 0, 1, 0, U, V, *dev)     # introduce the chain
+    H1, T1, Un, Vn = _host(Hd), _host(Td), _host(U), _host(V)
+    assert np.linalg.norm(Un.T @ H0 @ Vn - H1) / np.linalg.norm(H0) < RES_TOL
+    assert np.linalg.norm(Un.T @ T0 @ Vn - T1) / np.linalg.norm(T0) < RES_TOL
+    # GPU bulge b uses shift pair nb-1-b; the oracle numbers bulges from the top
+    idx = []
+    for b in range(nb):
+        q = nb - 1 - b
+        idx += [2 * q, 2 * q + 1]
+    oshifts = [np.ascontiguousarray(x[idx]) for x in (sr, si, sb)]
+    return torch, gqz, O, H1, T1, dev, oshifts
+
+
+@pytest.mark.parametrize("steps,last", [(10, 0), (64, 1)])
+def test_chase_window_matches_oracle(steps, last):
+    n, nb = 64, 5
+    _note("chase", steps, last)
+    torch, gqz, O, H1, T1, dev, osh = _chase_setup(n, nb, 7)
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H1)), gqz.to_colmajor(torch.from_numpy(T1))
+    U = gqz.colmajor(n); V = gqz.colmajor(n)
+    gqz.chase_window(Hd, Td, 0, n, nb, 0, steps, 0, last, U, V, *dev)
+    H2, T2, U2, V2 = _host(Hd), _host(Td), _host(U), _host(V)
+    Ho, To = O.fmat(H1), O.fmat(T1); Uo = O.fmat(np.eye(n)); Vo = O.fmat(np.eye(n))
+    O.lib().orc_chase_window(n, O.ptr(Ho), n, O.ptr(To), n, O.ptr(Uo), O.ptr(Vo), n, nb, steps, last,
+                             O.ptr(osh[0]), O.ptr(osh[1]), O.ptr(osh[2]))
+    # same reflector sequence: agreement to rounding (no re-ordering of operations beyond FMA use)
+    assert np.abs(H2 - Ho).max() / np.abs(Ho).max() < 1e-12
+    assert np.abs(T2 - To).max() / np.abs(To).max() < 1e-12
+    assert np.abs(U2 - Uo).max() < 1e-12 and np.abs(V2 - Vo).max() < 1e-12
+    if last:
+        assert np.abs(np.tril(H2, -2)).max() < 1e-12 * np.abs(H2).max()
+        assert np.abs(np.tril(T2, -1)).max() < 1e-12 * np.abs(T2).max()
+
+
+# ---- (KKQZ role) LDS-resident base QZ ------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 8, 40, 64])
+def test_small_qz(n):
+    torch, gqz, O = _mods()
+    _note("small_qz", n)
+    A = O.normal(10 + n, (n, n)); B = O.normal(20 + n, (n, n))
+    H, T, _, _ = O.ht_reduce(A, B)
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+    U = gqz.colmajor(n); V = gqz.colmajor(n)
+    ar = torch.zeros(n, dtype=torch.float64, device="cuda"); ai = torch.zeros_like(ar); be = torch.zeros_like(ar)
+    ttol = np.finfo(float).eps * np.linalg.norm(T)
+    info = gqz.small_qz(Hd, Td, U, V, ar, ai, be, ttol)
+    assert info == 0
+    S, P, Un, Vn = _host(Hd), _host(Td), _host(U), _host(V)
+    ra, rb, rq, rz = QC.compres(H, T, S, P, Un, Vn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    below, tb, ok2 = QC.schur_structure(S, P)
+    assert below == 0 and tb == 0 and ok2
+    ratio, _, _ = QC.match_eigs(_host(ar), _host(ai), _host(be), H, T)
+    assert ratio < 1.0
+
+
+# ---- (a) HT reduction ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [50, 200])
+def test_ht_reduction_device(n):
+    torch, gqz, O = _mods()
+    _note("ht", n)
+    A = O.normal(300 + n, (n, n)); B = O.normal(400 + n, (n, n))
+    Ad, Bd = gqz.to_colmajor(torch.from_numpy(A)), gqz.to_colmajor(torch.from_numpy(B))
+    Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+    gqz.geqrb("I", Ad, Bd, Qd)
+    gqz.pdgghrd("V", "I", Ad, Bd, Qd, Zd)
+    Hn, Tn, Qn, Zn = _host(Ad), _host(Bd), _host(Qd), _host(Zd)
+    ra, rb, rq, rz = QC.compres(A, B, Hn, Tn, Qn, Zn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    assert np.abs(np.tril(Hn, -2)).max() == 0 and np.abs(np.tril(Tn, -1)).max() == 0
+    # same orthogonal reduction as the oracle's restatement: H agrees up to column/row signs
+    Ho, To, _, _ = O.ht_reduce(A, B)
+    assert np.allclose(np.abs(Hn), np.abs(Ho), atol=1e-9 * np.abs(Ho).max())
+
+
+def test_pdgghrd_transposed_q_convention():
+    """PDGGHRD returns Q transposed relative to PDHGEQZ (PDGGHRD.f:25-27,83-91): with COMPQ='I'
+    the returned Q satisfies H = Q A Z."""
+    torch, gqz, O = _mods()
+    n = 60
+    _note("pdgghrd convention", n)
+    A = O.normal(91, (n, n)); B = np.triu(O.normal(92, (n, n)))
+    Fl = C.CDLL(os.path.join(os.path.dirname(gqz.lib_path()), "libgqz.so"))
+    H = np.asfortranarray(A.copy()); T = np.asfortranarray(B.copy())
+    Q = np.asfortranarray(np.zeros((n, n))); Z = np.asfortranarray(np.zeros((n, n)))
+    d = (C.c_int * 9)(1, 0, n, n, 64, 64, 0, 0, n)
+    w = np.zeros(n * 64)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    Fl.gqz_pdgghrd_host.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int]
+    info = Fl.gqz_pdgghrd_host(b"I", b"I", n, 1, n, vp(H), d, vp(T), d, vp(Q), d, vp(Z), d, vp(w), len(w))
+    assert info == 0
+    assert np.linalg.norm(Q @ A @ Z - H) / np.linalg.norm(A) < RES_TOL
+    assert np.linalg.norm(Q @ B @ Z - T) / np.linalg.norm(B) < RES_TOL
+    assert np.abs(np.tril(H, -2)).max() == 0 and np.abs(np.tril(T, -1)).max() == 0
+
+
+# ---- full multishift QZ from HT form: GPU vs oracle vs LAPACK ------------------------------------
+@pytest.mark.parametrize("n", [80, 300, 700])
+def test_hgeqz_vs_oracle_and_lapack(n):
+    torch, gqz, O = _mods()
+    _note("hgeqz", n)
+    A = O.normal(100 + n, (n, n)); B = O.normal(200 + n, (n, n))
+    H, T, Q, Z = O.ht_reduce(A, B)
+    Hd, Td, Qd, Zd = (gqz.to_colmajor(torch.from_numpy(x)) for x in (H, T, Q, Z))
+    ar, ai, be, info, st = gqz.pdhgeqz("S", "V", "V", Hd, Td, Qd, Zd)
+    assert info == 0
+    S, P, Qn, Zn = _host(Hd), _host(Td), _host(Qd), _host(Zd)
+    ra, rb, rq, rz = QC.compres(A, B, S, P, Qn, Zn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    below, tb, ok2 = QC.schur_structure(S, P)
+    assert below == 0 and tb == 0 and ok2
+    ar, ai, be = _host(ar), _host(ai), _host(be)
+    assert (be >= 0).all()
+    # the reported eigenvalues are those of the Schur form's diagonal blocks
+    j = 0
+    while j < n:
+        if j + 1 < n and S[j + 1, j] != 0:
+            j += 2
+        else:
+            assert ar[j] == S[j, j] and be[j] == P[j, j] and ai[j] == 0
+            j += 1
+    if n <= 300:
+        ratio, _, _ = QC.match_eigs(ar, ai, be, A, B)
+        assert ratio < 1.0
+    # the oracle on the same HT pencil: same eigenvalue set (chordal, condition-weighted)
+    _, _, oar, oai, obe, _, _, oinfo, _ = O.hgeqz(H, T)
+    assert oinfo == 0
+    ratio2, _, _ = QC.match_eigs(ar, ai, be, H, T) if n <= 300 else (0.0, 0, 0)
+    assert ratio2 < 1.0
+    a1 = ar + 1j * ai; a2 = oar + 1j * oai
+    D = QC.chordal(a1[:, None], be[:, None], a2[None, :], obe[None, :])
+    from scipy.optimize import linear_sum_assignment
+    r, c = linear_sum_assignment(D)
+    assert np.median(D[r, c]) < 1e-10
+
+
+def test_hgeqz_paths_with_aed_and_multishift():
+    """n=1500 runs the multishift sweep with AED (NS=64, NW=64 by PIPARMQ) end to end."""
+    torch, gqz, O = _mods()
+    n = 1500
+    _note("hgeqz multishift", n)
+    H, T = QC.pencil_ht_random(n, 11)
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+    Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+    ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", Hd, Td, Qd, Zd)
+    assert info == 0 and st["aed_calls"] > 0 and st["sweeps"] > 0
+    ra, rb, rq, rz = gqz.compres(gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T)),
+                                 Hd, Td, Qd, Zd)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+
+
+# ---- call-surface variants (JOB / COMPQ / COMPZ / ILO-IHI) ----------------------------------------
+def test_job_e_and_compq_n():
+    torch, gqz, O = _mods()
+    n = 150
+    _note("job E / compq N", n)
+    H, T = QC.pencil_ht_random(n, 3)
+    ref = None
+    for job, cq, cz in (("E", "N", "N"), ("S", "N", "I"), ("S", "I", "N")):
+        Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+        Qd = gqz.colmajor(n) if cq != "N" else None
+        Zd = gqz.colmajor(n) if cz != "N" else None
+        ar, ai, be, info, _ = gqz.pdhgeqz(job, cq, cz, Hd, Td, Qd, Zd)
+        assert info == 0
+        ratio, _, _ = QC.match_eigs(_host(ar), _host(ai), _host(be), H, T)
+        assert ratio < 1.0
+        for M in (Qd, Zd):
+            if M is not None:
+                Mn = _host(M)
+                assert np.linalg.norm(Mn.T @ Mn - np.eye(n)) / (n * QC.EPS) < ORTH_TOL
+
+
+def test_ilo_ihi_subblock():
+    """Active block [ILO, IHI] of a pencil already triangular outside it (PDHGEQZ.f:132-161)."""
+    torch, gqz, O = _mods()
+    n, ilo, ihi = 120, 11, 100
+    _note("ilo/ihi", n, ilo, ihi)
+    H, T = QC.pencil_ht_random(n, 5)
+    H[ilo - 1, ilo - 2] = 0.0
+    H[ihi, ihi - 1] = 0.0
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+    Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+    ar, ai, be, info, _ = gqz.pdhgeqz("S", "I", "I", Hd, Td, Qd, Zd, ilo=ilo, ihi=ihi)
+    assert info == 0
+    S, P, Qn, Zn = _host(Hd), _host(Td), _host(Qd), _host(Zd)
+    ra, rb, rq, rz = QC.compres(H, T, S, P, Qn, Zn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    # transforms act on rows/cols of the block only
+    I = np.eye(n)
+    assert np.array_equal(Qn[:ilo - 1], I[:ilo - 1]) and np.array_equal(Qn[ihi:], I[ihi:])
+    b = slice(ilo - 1, ihi)
+    ratio, _, _ = QC.match_eigs(_host(ar)[b], _host(ai)[b], _host(be)[b], H[b, b], T[b, b])
+    assert ratio < 1.0
+
+
+# ---- infinite eigenvalues ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,ninf", [(150, 7), (400, 20)])
+def test_known_spectrum_and_infinite_count(n, ninf):
+    torch, gqz, O = _mods()
+    _note("known", n, ninf)
+    A, B, fin, k = QC.pencil_known(n, 17 + n, n_inf=ninf)
+    Ad, Bd = gqz.to_colmajor(torch.from_numpy(A)), gqz.to_colmajor(torch.from_numpy(B))
+    S, P, Q, Z, ar, ai, be, info, _ = gqz.schur_pipeline(Ad, Bd)
+    assert info == 0
+    be = _host(be)
+    assert int((be == 0).sum()) == k
+    ra, rb, rq, rz = QC.compres(A, B, _host(S), _host(P), _host(Q), _host(Z))
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    ev = (_host(ar) + 1j * _host(ai))[be != 0] / be[be != 0]
+    from scipy.optimize import linear_sum_assignment
+    D = np.abs(ev[:, None] - fin[None, :]) / np.abs(fin[None, :])
+    r, c = linear_sum_assignment(D)
+    assert D[r, c].max() < 1e-8
+
+
+def test_stress_pencil_infinite_count_matches_oracle():
+    """Stress family (sparse + 0.5 I, zero diagonal entries in triangular B, graded): the GPU QZ
+    and the oracle, run on the same HT pencil, flag the same number of infinite eigenvalues."""
+    torch, gqz, O = _mods()
+    n = 240
+    _note("stress", n)
+    A, B = QC.pencil_stress(n, 4)
+    H, T, Q, Z = O.ht_reduce(A, B)
+    Hd, Td, Qd, Zd = (gqz.to_colmajor(torch.from_numpy(x)) for x in (H, T, Q, Z))
+    ar, ai, be, info, _ = gqz.pdhgeqz("S", "V", "V", Hd, Td, Qd, Zd)
+    assert info == 0
+    _, _, oar, oai, obe, _, _, oinfo, _ = O.hgeqz(H, T)
+    assert oinfo == 0
+    assert int((_host(be) == 0).sum()) == int((obe == 0).sum())
+    ra, rb, rq, rz = QC.compres(A, B, _host(Hd), _host(Td), _host(Qd), _host(Zd))
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+
+
+# ---- Fortran call surface on the GPU -----------------------------------------------------------------
+def test_fortran_pdhgeqz_on_gpu():
+    torch, gqz, O = _mods()
+    flib = os.path.join(os.path.dirname(gqz.lib_path()), "libpdhgeqz_f.so")
+    if not os.path.exists(flib):
+        pytest.skip("Fortran shim not built")
+    F = C.CDLL(flib)
+    n = 100
+    _note("fortran pdhgeqz", n)
+    A = O.normal(61, (n, n)); B = O.normal(62, (n, n))
+    H0, T0, Q0, Z0 = O.ht_reduce(A, B)
+    H, T, Q, Z = (np.asfortranarray(x.copy()) for x in (H0, T0, Q0, Z0))
+    ar = np.zeros(n); ai = np.zeros(n); be = np.zeros(n)
+    nb = 60
+    d = (C.c_int * 9)(1, 0, n, n, nb, nb, 0, 0, n)
+    dw = np.zeros(n * nb); iw = np.zeros(4 * n, dtype=np.int32)
+    ic = lambda v: C.byref(C.c_int(v))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    info = C.c_int(-999)
+    F.kkqzconf_(C.byref(C.c_double(np.finfo(float).eps)))
+    F.pdhgeqz_(C.c_char_p(b"S"), C.c_char_p(b"V"), C.c_char_p(b"V"), ic(n), ic(1), ic(n), vp(H), d, vp(T), d,
+               vp(ar), vp(ai), vp(be), vp(Q), d, vp(Z), d, vp(dw), ic(len(dw)), vp(iw), ic(len(iw)),
+               C.byref(info), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1))
+    assert info.value == 0
+    ra, rb, rq, rz = QC.compres(A, B, H, T, Q, Z)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    ratio, _, _ = QC.match_eigs(ar, ai, be, A, B)
+    assert ratio < 1.0
+
+
+# ---- batched pencils ---------------------------------------------------------------------------------
+def _batch(ns, seed):
+    rng = np.random.default_rng(seed)
+    A = [rng.standard_normal((k, k)) for k in ns]
+    B = [rng.standard_normal((k, k)) for k in ns]
+    off = np.concatenate([[0], np.cumsum([k * k for k in ns])[:-1]]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    flat = lambda Ms: np.concatenate([M.ravel(order="F") for M in Ms])
+    return A, B, off, eoff, flat(A), flat(B)
+
+
+def _run_batched(gqz, torch, ns, fA, fB, off, eoff, nt=0):
+    tot = int(sum(k * k for k in ns)); et = int(sum(ns))
+    dA = torch.from_numpy(fA).cuda(); dB = torch.from_numpy(fB).cuda()
+    dQ = torch.empty(tot, dtype=torch.float64, device="cuda"); dZ = torch.empty_like(dQ)
+    ar = torch.zeros(et, dtype=torch.float64, device="cuda"); ai = torch.zeros_like(ar); be = torch.zeros_like(ar)
+    info = torch.full((len(ns),), -7, dtype=torch.int32, device="cuda")
+    L = gqz.lib()
+    nn = np.ascontiguousarray(ns, dtype=np.int32)
+    torch.cuda.synchronize()
+    r = L.gqz_batched_dev(len(ns), nn.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                          eoff.ctypes.data_as(C.c_void_p), C.c_void_p(dA.data_ptr()), C.c_void_p(dB.data_ptr()),
+                          C.c_void_p(dQ.data_ptr()), C.c_void_p(dZ.data_ptr()), C.c_void_p(ar.data_ptr()),
+                          C.c_void_p(ai.data_ptr()), C.c_void_p(be.data_ptr()), C.c_void_p(info.data_ptr()), nt)
+    assert r == 0, gqz.lib().gqz_last_error()
+    return [_host(x) for x in (dA, dB, dQ, dZ, ar, ai, be, info)]
+
+
+@pytest.mark.parametrize("nt", [0, 128, 512])
+def test_batched_pencils(nt):
+    torch, gqz, O = _mods()
+    rng = np.random.default_rng(5)
+    ns = [int(round(8 * 16 ** u)) for u in rng.uniform(0, 1, 48)] + [1, 2, 3]
+    _note("batched", len(ns), "max n", max(ns), "threads", nt)
+    A, B, off, eoff, fA, fB = _batch(ns, 9)
+    S, P, Q, Z, ar, ai, be, info = _run_batched(gqz, torch, ns, fA, fB, off, eoff, nt)
+    assert (info == 0).all()
+    for i, k in enumerate(ns):
+        o, e = off[i], eoff[i]
+        M = lambda x: x[o:o + k * k].reshape(k, k, order="F")
+        ra, rb, rq, rz = QC.compres(A[i], B[i], M(S), M(P), M(Q), M(Z))
+        assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL, (i, k, ra, rb, rq, rz)
+        below, tb, ok2 = QC.schur_structure(M(S), M(P))
+        assert below == 0 and tb == 0 and ok2
+        ratio, _, _ = QC.match_eigs(ar[e:e + k], ai[e:e + k], be[e:e + k], A[i], B[i])
+        assert ratio < 1.0, (i, k, ratio)
+
+
+def test_batched_graph_replay_and_tuner():
+    """A second call with the same arguments replays the captured graph and gives bit-identical
+    results; the autotuner picks a valid launch shape."""
+    torch, gqz, O = _mods()
+    rng = np.random.default_rng(8)
+    ns = [int(round(16 * 8 ** u)) for u in rng.uniform(0, 1, 24)]
+    _note("batched replay", len(ns))
+    A, B, off, eoff, fA, fB = _batch(ns, 3)
+    L = gqz.lib()
+    nn = np.ascontiguousarray(ns, dtype=np.int32)
+    bt = C.c_int(0); bw = C.c_int(0); bms = C.c_double(0)
+    r = L.gqz_batched_tune(len(ns), nn.ctypes.data_as(C.c_void_p), C.byref(bt), C.byref(bw), C.byref(bms))
+    assert r == 0 and bt.value in (128, 256, 512, 1024) and bw.value > 0 and bms.value > 0
+    tot = int(sum(k * k for k in ns)); et = int(sum(ns))
+    dA0 = torch.from_numpy(fA).cuda(); dB0 = torch.from_numpy(fB).cuda()
+    dA = dA0.clone(); dB = dB0.clone()
+    dQ = torch.empty(tot, dtype=torch.float64, device="cuda"); dZ = torch.empty_like(dQ)
+    ar = torch.zeros(et, dtype=torch.float64, device="cuda"); ai = torch.zeros_like(ar); be = torch.zeros_like(ar)
+    info = torch.zeros(len(ns), dtype=torch.int32, device="cuda")
+    args = lambda: (len(ns), nn.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                    eoff.ctypes.data_as(C.c_void_p), C.c_void_p(dA.data_ptr()), C.c_void_p(dB.data_ptr()),
+                    C.c_void_p(dQ.data_ptr()), C.c_void_p(dZ.data_ptr()), C.c_void_p(ar.data_ptr()),
+                    C.c_void_p(ai.data_ptr()), C.c_void_p(be.data_ptr()), C.c_void_p(info.data_ptr()), 0)
+    outs = []
+    for rep in range(3):
+        dA.copy_(dA0); dB.copy_(dB0)
+        torch.cuda.synchronize()
+        assert L.gqz_batched_dev(*args()) == 0
+        outs.append([_host(x).copy() for x in (dA, dB, dQ, dZ, ar, ai, be, info)])
+    for rep in (1, 2):
+        for a, b in zip(outs[0], outs[rep]):
+            assert np.array_equal(a, b)
+    assert (outs[0][7] == 0).all()
