@@ -127,10 +127,13 @@ __device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &
     w0 -= s2 * x2[0]; w1 -= s2;
     double s1 = tau1 * (w0 * u1[0] + w1 * u1[1] + w2);
     w0 -= s1 * u1[0]; w1 -= s1 * u1[1]; w2 -= s1;
-    double al = w0;
-    double xx[2] = { w1, w2 };
-    d_larfg(3, al, xx, 1, tau);
-    v[0] = 1.0; v[1] = xx[0]; v[2] = xx[1];
+    // (w0, w1, w2) is a unit vector: reflect it onto -sign(w0) e0 with v = w + sign(w0) e0 scaled to
+    // v0 = 1 and tau = 2 / (v^T v) from v itself; a norm-based DLARFG on a unit vector rounds the norm
+    // to 1 with a bias that makes the accumulated Z lose orthogonality coherently
+    double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
+    double p1 = w1 / p0, p2 = w2 / p0;
+    tau = 2.0 / (1.0 + p1 * p1 + p2 * p2);
+    v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
 // small dense helpers on column-major m x m (m <= 4) arrays

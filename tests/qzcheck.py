@@ -136,3 +136,25 @@ def match_eigs(ar, ai, be, A, B, fudge=100.0):
     tol = fudge * n * EPS * nab * cond[ci]
     ratio = D[ri, ci] / np.maximum(tol, 1e-300)
     return float(ratio.max()), float(D[ri, ci].max()), int((np.asarray(be) == 0).sum())
+
+
+def match_known(ar, ai, be, known, A, B, fudge=100.0):
+    """Finite computed eigenvalues against a known finite spectrum, with the same condition-weighted
+    chordal tolerance as match_eigs (condition numbers from LAPACK's eigenvectors of (A, B), matched
+    to the known values).  Returns the worst ratio distance / tolerance."""
+    n = A.shape[0]
+    be = np.asarray(be)
+    fin = be != 0
+    ev = (np.asarray(ar) + 1j * np.asarray(ai))[fin] / be[fin]
+    a2, b2, cond = lapack_eigs(A, B)
+    lf = np.abs(b2) > 0
+    lam = a2[lf] / b2[lf]
+    kap = cond[lf]
+    # condition number of each known eigenvalue: that of the nearest LAPACK eigenvalue
+    D0 = np.abs(known[:, None] - lam[None, :])
+    kk = kap[np.argmin(D0, axis=1)]
+    D = chordal(ev[:, None], np.ones((len(ev), 1)), known[None, :].astype(complex), np.ones((1, len(known))))
+    ri, ci = linear_sum_assignment(D)
+    nab = np.sqrt(np.linalg.norm(A) ** 2 + np.linalg.norm(B) ** 2)
+    tol = fudge * n * EPS * nab * kk[ci]
+    return float((D[ri, ci] / tol).max())
