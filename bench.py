@@ -131,8 +131,8 @@ def main():
         log("step %d: n=%d %.3f s info=%d sweeps=%d aed=%d" % (s, n, times[-1], info, st["sweeps"], st["aed_calls"]))
     torch.cuda.synchronize()
     t_all = time.perf_counter() - t_all0
-    kt = (KTime * 8)()
-    nk = L.gqz_ktime_get(kt, 8)
+    kt = (KTime * 16)()
+    nk = L.gqz_ktime_get(kt, 16)
     L.gqz_ktime_enable(0)
     classes = [dict(name=kt[i].name.decode(), launches=kt[i].launches, ms=kt[i].ms, flops=kt[i].flops,
                     bytes=kt[i].bytes) for i in range(max(nk, 0))]

@@ -23,19 +23,24 @@ extern "C" __global__ void k_gghrd_wg(double *A, double *B, int ld, int n, int i
                                       double *Q, double *Z, int ldq, int nq);
 #endif
 
-// up to four in-place slab updates sharing one launch (same k): C_q <- C_q F_q (right) or
-// C_q <- F_q^T C_q (left); m[q] = rows (right) or columns (left) of slab q.
+// in-place slab updates sharing one launch: C_q <- C_q F_q (right) or C_q <- F_q^T C_q (left);
+// m[q] = rows (right) or columns (left) of slab q, k[q] = order of its factor F_q.  Slabs of one
+// launch must not overlap.
+#define GQZ_SLAB_MAXJOBS 64
 struct GqzSlabJobs {
     int njobs;
-    double *C[4];
-    int ldc[4];
-    int m[4];
-    const double *F[4];
-    int ldf[4];
-    int nblk[4];
+    double *C[GQZ_SLAB_MAXJOBS];
+    const double *F[GQZ_SLAB_MAXJOBS];
+    int ldc[GQZ_SLAB_MAXJOBS];
+    int m[GQZ_SLAB_MAXJOBS];
+    int ldf[GQZ_SLAB_MAXJOBS];
+    int k[GQZ_SLAB_MAXJOBS];
+    int blk0[GQZ_SLAB_MAXJOBS + 1];   // first workgroup of each job (filled by the launcher)
 };
-void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs J, int k);
-void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs J, int k);
+// add a slab to a job list (ignored when empty); returns false when the list is full
+bool gqz_slab_add(GqzSlabJobs &J, double *C, int ldc, int m, const double *F, int ldf, int k);
+void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs &J);
+void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs &J);
 
 // ---- host launchers (defined in the .hip files; all asynchronous on `st`) -----------------
 // C[m x k] <- C * V[k x k]   (in place, k <= 1024)
@@ -64,4 +69,18 @@ struct GqzChainArgs {
     double smlnum;
 };
 void gqz_chase_window(hipStream_t st, const GqzChainArgs &a);
+
+// several chains chased at once, one workgroup each, in disjoint diagonal windows
+#define GQZ_MAXCH 16
+struct GqzChainSet {
+    double *H, *T;
+    int ld, nch;
+    double *U, *V;            // chain c: U + c * ustride (w_c x w_c, ld w_c); same for V
+    int ustride, pad;
+    const double *sr, *si, *sb;
+    double smlnum;
+    int p[GQZ_MAXCH], w[GQZ_MAXCH], nb[GQZ_MAXCH], t0[GQZ_MAXCH], steps[GQZ_MAXCH];
+    int intro[GQZ_MAXCH], last[GQZ_MAXCH], soff[GQZ_MAXCH];
+};
+void gqz_chase_set(hipStream_t st, const GqzChainSet &s);
 int gqz_chase_max_window();

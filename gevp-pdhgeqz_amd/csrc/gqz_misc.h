@@ -24,8 +24,11 @@ void gqz_launch_sub(hipStream_t st, double *R, int ldr, const double *S, int lds
 // AED window post-processing (k_aed.hip); pscr holds (jw/2 + 1) * 65 doubles
 void gqz_launch_aed_reorder(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
                             const double *sptr, double smlnum, double *pscr, int *out);
+void gqz_launch_aed_eigs(hipStream_t st, const double *Hw, const double *Tw, int jw, double *war, double *wai,
+                         double *wb);
+// spike reflector + RQ update of T (scr: 4 * jw doubles); reads kbc from out[0]
 void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
-                          const double *sptr, double *war, double *wai, double *wb, double *scr, const int *out);
+                          const double *sptr, double *scr, const int *out);
 
 // library state shared with entry points living beside their kernels (qz_host.cpp)
 int gqz_init_current();                 // initialise on device 0 if needed and make it current; 0 = ok

@@ -191,16 +191,10 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
     }
 }
 
-// eigenvalues of the window blocks + spike reflector + RQ of T(0:kbc, 0:kbc)
-extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, double *Tw, double *U, double *V,
-                                                               int jw, const double *sptr, double *war,
-                                                               double *wai, double *wb, double *scr,
-                                                               const int *out) {
-    __shared__ double s_tau;
-    const int tid = threadIdx.x;
-    const double s = sptr ? *sptr : 0.0;
-    const int kbc = out[0];
-    for (int j = tid; j < jw; j += blockDim.x) {
+// eigenvalues of the window's diagonal blocks (the shifts and the deflated eigenvalues)
+extern "C" __global__ void __launch_bounds__(256) k_aed_eigs(const double *Hw, const double *Tw, int jw,
+                                                             double *war, double *wai, double *wb) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jw; j += gridDim.x * blockDim.x) {
         const bool lower2 = (j > 0) && Hw[GQZ_IX(j, j - 1, jw)] != 0.0;
         const bool upper2 = (j + 1 < jw) && Hw[GQZ_IX(j + 1, j, jw)] != 0.0;
         if (lower2) continue;
@@ -221,61 +215,141 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
             war[j] = Hw[GQZ_IX(j, j, jw)]; wai[j] = 0.0; wb[j] = Tw[GQZ_IX(j, j, jw)];
         }
     }
+}
+
+// Spike reflector and RQ update (qzearly.f:372-436 with PARA_AGGRQUP = .TRUE., kkqzconf.f:138):
+//   P = I - tau v v^T maps the spike s U(0, 0:kbc) onto a multiple of e_0 and is applied to the
+//   rows 0..kbc-1 of Hw (all columns) and T's columns right of the block, and to U's columns.
+//   P T(0:kbc, 0:kbc) = T + v w^T with w = -tau T^T v is brought back to triangular form by
+//   2 kbc - 2 column rotations: the first sweep (i = 0 .. kbc-2) folds w into its last entry
+//   (T becomes Hessenberg, the rank-one term lives in the last column), the second sweep
+//   (i = kbc-2 .. 0) removes the subdiagonal.  Hw's block columns and V see the same rotations.
+// One workgroup; kbc <= AED_MAXW.  Rotations are kept in scr (4 kbc doubles).
+extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, double *Tw, double *U, double *V,
+                                                               int jw, const double *sptr, double *scr,
+                                                               const int *out) {
+    __shared__ double sv[AED_MAXW], sw[AED_MAXW];
+    __shared__ double red[1024];
+    __shared__ double s_tau, s_cs, s_sn;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double s = sptr ? *sptr : 0.0;
+    const int kbc = out[0];
+    if (kbc <= 0 || s == 0.0 || kbc == jw) return;
+    // ---- reflector (DLARFG on s U(0, 0:kbc)) ----
+    for (int i = tid; i < kbc; i += nt) sv[i] = s * U[GQZ_IX(0, i, jw)];
     __syncthreads();
-    if (kbc <= 0 || s == 0.0) return;
-    double *vsp = scr;
+    double ss = 0.0;
+    for (int i = 1 + tid; i < kbc; i += nt) ss += sv[i] * sv[i];
+    red[tid] = ss;
+    __syncthreads();
+    for (int h = nt / 2; h > 0; h >>= 1) { if (tid < h) red[tid] += red[tid + h]; __syncthreads(); }
     if (tid == 0) {
-        for (int i = 0; i < kbc; ++i) vsp[i] = s * U[GQZ_IX(0, i, jw)];
-        double alpha = vsp[0], tau;
-        d_larfg(kbc, alpha, vsp + 1, 1, tau);
-        vsp[0] = 1.0;
-        s_tau = tau;
-    }
-    __syncthreads();
-    {
-        const double tau = s_tau;
-        for (int idx = tid; idx < 3 * jw; idx += blockDim.x) {
-            if (idx < 2 * jw) {
-                double *M = idx < jw ? Hw : Tw;
-                const int col = idx < jw ? idx : idx - jw;
-                double *p = M + GQZ_IX(0, col, jw);
-                double t = 0.0;
-                for (int i = 0; i < kbc; ++i) t += vsp[i] * p[i];
-                t *= tau;
-                for (int i = 0; i < kbc; ++i) p[i] -= t * vsp[i];
-            } else {
-                const int row = idx - 2 * jw;
-                double t = 0.0;
-                for (int j = 0; j < kbc; ++j) t += U[GQZ_IX(row, j, jw)] * vsp[j];
-                t *= tau;
-                for (int j = 0; j < kbc; ++j) U[GQZ_IX(row, j, jw)] -= t * vsp[j];
-            }
+        const double xnorm = sqrt(red[0]), alpha = sv[0];
+        if (xnorm == 0.0) { s_tau = 0.0; s_cs = 1.0; }
+        else {
+            const double beta = -copysign(hypot(alpha, xnorm), alpha);
+            s_tau = (beta - alpha) / beta;
+            s_cs = 1.0 / (alpha - beta);
         }
     }
     __syncthreads();
-    for (int r = kbc - 1; r >= 1; --r) {
-        if (tid == 0) {
-            for (int j = 0; j < r; ++j) vsp[j] = Tw[GQZ_IX(r, j, jw)];
-            double alpha = Tw[GQZ_IX(r, r, jw)], tau;
-            d_larfg(r + 1, alpha, vsp, 1, tau);
-            vsp[r] = 1.0;
-            Tw[GQZ_IX(r, r, jw)] = alpha;
-            for (int j = 0; j < r; ++j) Tw[GQZ_IX(r, j, jw)] = 0.0;
-            s_tau = tau;
-        }
-        __syncthreads();
-        const double tau = s_tau;
-        for (int idx = tid; idx < r + 2 * jw; idx += blockDim.x) {
-            double *M; int row;
-            if (idx < r) { M = Tw; row = idx; }
-            else if (idx < r + jw) { M = Hw; row = idx - r; }
-            else { M = V; row = idx - r - jw; }
+    const double tau = s_tau;
+    if (tau == 0.0) return;
+    for (int i = 1 + tid; i < kbc; i += nt) sv[i] *= s_cs;
+    if (tid == 0) sv[0] = 1.0;
+    __syncthreads();
+    // ---- apply P: H rows (all columns), T rows (columns right of the block), U columns; and
+    //      w = -tau T(0:kbc,0:kbc)^T v (T upper triangular) ----
+    for (int idx = tid; idx < jw + (jw - kbc) + jw + kbc; idx += nt) {
+        if (idx < jw + (jw - kbc)) {
+            double *p = (idx < jw) ? Hw + GQZ_IX(0, idx, jw) : Tw + GQZ_IX(0, kbc + idx - jw, jw);
             double t = 0.0;
-            for (int j = 0; j <= r; ++j) t += M[GQZ_IX(row, j, jw)] * vsp[j];
+            for (int i = 0; i < kbc; ++i) t += sv[i] * p[i];
             t *= tau;
-            for (int j = 0; j <= r; ++j) M[GQZ_IX(row, j, jw)] -= t * vsp[j];
+            for (int i = 0; i < kbc; ++i) p[i] -= t * sv[i];
+        } else if (idx < 3 * jw - kbc) {
+            const int row = idx - (2 * jw - kbc);
+            double t = 0.0;
+            for (int j = 0; j < kbc; ++j) t += U[GQZ_IX(row, j, jw)] * sv[j];
+            t *= tau;
+            for (int j = 0; j < kbc; ++j) U[GQZ_IX(row, j, jw)] -= t * sv[j];
+        } else {
+            const int j = idx - (3 * jw - kbc);
+            double t = 0.0;
+            for (int i = 0; i <= j; ++i) t += Tw[GQZ_IX(i, j, jw)] * sv[i];
+            sw[j] = -tau * t;
+        }
+    }
+    __syncthreads();
+    // ---- first sweep: rotations (i, i+1) fold w into w(kbc-1) (qzearly.f:406-412) ----
+    double *rc = scr, *rs = scr + kbc, *qc = scr + 2 * kbc, *qs = scr + 3 * kbc;
+    if (tid == 0) {
+        for (int i = 0; i < kbc - 1; ++i) {
+            double c, sn, r;
+            d_lartg(sw[i + 1], sw[i], c, sn, r);
+            sw[i + 1] = r;
+            rc[i] = c; rs[i] = -sn;
+        }
+    }
+    __syncthreads();
+    // T columns (i, i+1), rows 0..i+1, in order i = 0..kbc-2: each row applies the rotations that
+    // reach it (rotation i touches rows <= i + 1) in order
+    for (int row = tid; row < kbc; row += nt) {
+        const int i0 = max(0, row - 1);
+        if (i0 >= kbc - 1) continue;
+        double x = Tw[GQZ_IX(row, i0, jw)];
+        for (int i = i0; i < kbc - 1; ++i) {
+            double y = Tw[GQZ_IX(row, i + 1, jw)];
+            Tw[GQZ_IX(row, i, jw)] = rc[i] * x + rs[i] * y;
+            x = rc[i] * y - rs[i] * x;
+        }
+        Tw[GQZ_IX(row, kbc - 1, jw)] = x;
+    }
+    __syncthreads();
+    // rank-one term now lives in the last column: T(:, kbc-1) += w(kbc-1) v
+    for (int i = tid; i < kbc; i += nt) Tw[GQZ_IX(i, kbc - 1, jw)] += sw[kbc - 1] * sv[i];
+    // H block columns (rows 0..kbc-1; H below row kbc is zero in these columns) and V (all rows)
+    for (int row = tid; row < kbc + jw; row += nt) {
+        double *M = row < kbc ? Hw : V;
+        const int r = row < kbc ? row : row - kbc;
+        double x = M[GQZ_IX(r, 0, jw)];
+        for (int i = 0; i < kbc - 1; ++i) {
+            double y = M[GQZ_IX(r, i + 1, jw)];
+            M[GQZ_IX(r, i, jw)] = rc[i] * x + rs[i] * y;
+            x = rc[i] * y - rs[i] * x;
+        }
+        M[GQZ_IX(r, kbc - 1, jw)] = x;
+    }
+    __syncthreads();
+    // ---- second sweep: i = kbc-2 .. 0 removes T(i+1, i) (qzearly.f:421-429) ----
+    for (int i = kbc - 2; i >= 0; --i) {
+        if (tid == 0) {
+            double c, sn, r;
+            d_lartg(Tw[GQZ_IX(i + 1, i + 1, jw)], Tw[GQZ_IX(i + 1, i, jw)], c, sn, r);
+            Tw[GQZ_IX(i + 1, i + 1, jw)] = r;
+            Tw[GQZ_IX(i + 1, i, jw)] = 0.0;
+            qc[i] = c; qs[i] = -sn;
+            s_cs = c; s_sn = -sn;
         }
         __syncthreads();
+        const double c = s_cs, sn = s_sn;
+        for (int row = tid; row <= i; row += nt) {
+            double x = Tw[GQZ_IX(row, i, jw)], y = Tw[GQZ_IX(row, i + 1, jw)];
+            Tw[GQZ_IX(row, i, jw)] = c * x + sn * y;
+            Tw[GQZ_IX(row, i + 1, jw)] = c * y - sn * x;
+        }
+        __syncthreads();
+    }
+    for (int row = tid; row < kbc + jw; row += nt) {
+        double *M = row < kbc ? Hw : V;
+        const int r = row < kbc ? row : row - kbc;
+        double y = M[GQZ_IX(r, kbc - 1, jw)];
+        for (int i = kbc - 2; i >= 0; --i) {
+            double x = M[GQZ_IX(r, i, jw)];
+            M[GQZ_IX(r, i + 1, jw)] = qc[i] * y - qs[i] * x;
+            y = qc[i] * x + qs[i] * y;
+        }
+        M[GQZ_IX(r, 0, jw)] = y;
     }
 }
 
@@ -283,7 +357,11 @@ void gqz_launch_aed_reorder(hipStream_t st, double *Hw, double *Tw, double *U, d
                             const double *sptr, double smlnum, double *pscr, int *out) {
     hipLaunchKernelGGL(k_aed_reorder, dim3(1), dim3(256), 0, st, Hw, Tw, U, V, jw, sptr, smlnum, pscr, out);
 }
+void gqz_launch_aed_eigs(hipStream_t st, const double *Hw, const double *Tw, int jw, double *war, double *wai,
+                         double *wb) {
+    hipLaunchKernelGGL(k_aed_eigs, dim3((jw + 255) / 256), dim3(256), 0, st, Hw, Tw, jw, war, wai, wb);
+}
 void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
-                          const double *sptr, double *war, double *wai, double *wb, double *scr, const int *out) {
-    hipLaunchKernelGGL(k_aed_spike, dim3(1), dim3(1024), 0, st, Hw, Tw, U, V, jw, sptr, war, wai, wb, scr, out);
+                          const double *sptr, double *scr, const int *out) {
+    hipLaunchKernelGGL(k_aed_spike, dim3(1), dim3(1024), 0, st, Hw, Tw, U, V, jw, sptr, scr, out);
 }

@@ -24,8 +24,15 @@
 #define CH_MAXW 64
 #define CH_MAXB 21
 
-__global__ void __launch_bounds__(256) k_chase(GqzChainArgs a) {
+__global__ void __launch_bounds__(256) k_chase(GqzChainSet S) {
     extern __shared__ double lds[];
+    const int cidx = blockIdx.x;
+    GqzChainArgs a;
+    a.H = S.H; a.T = S.T; a.ld = S.ld; a.p = S.p[cidx]; a.w = S.w[cidx]; a.nb = S.nb[cidx];
+    a.t0 = S.t0[cidx]; a.steps = S.steps[cidx]; a.intro = S.intro[cidx]; a.last = S.last[cidx];
+    a.U = S.U + (size_t)cidx * S.ustride; a.V = S.V + (size_t)cidx * S.ustride;
+    a.sr = S.sr + S.soff[cidx]; a.si = S.si + S.soff[cidx]; a.sb = S.sb + S.soff[cidx];
+    a.smlnum = S.smlnum;
     const int w = a.w, ld = w + 1, nb = a.nb;
     double *sH = lds, *sT = sH + (size_t)ld * w, *sU = sT + (size_t)ld * w, *sV = sU + (size_t)ld * w;
     double *bv = sV + (size_t)ld * w;          // per bulge: v1 v2 tau w1 w2 tau2 c s  (8)
@@ -237,8 +244,29 @@ __global__ void __launch_bounds__(256) k_chase(GqzChainArgs a) {
 
 int gqz_chase_max_window() { return CH_MAXW; }
 
+static size_t chase_lds(int w) {
+    return sizeof(double) * (4 * (size_t)(w + 1) * w + 8 * CH_MAXB) + sizeof(int) * 2 * CH_MAXB;
+}
+
+void gqz_chase_set(hipStream_t st, const GqzChainSet &s) {
+    if (s.nch <= 0) return;
+    int wm = 0;
+    for (int c = 0; c < s.nch; ++c) wm = max(wm, s.w[c]);
+    size_t lds = chase_lds(wm);
+    static size_t lds_set = 0;
+    if (lds > lds_set) {
+        hipFuncSetAttribute((const void *)k_chase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(k_chase, dim3(s.nch), dim3(256), lds, st, s);
+}
+
 void gqz_chase_window(hipStream_t st, const GqzChainArgs &a) {
-    size_t lds = sizeof(double) * (4 * (size_t)(a.w + 1) * a.w + 8 * CH_MAXB) + sizeof(int) * 2 * CH_MAXB;
-    hipFuncSetAttribute((const void *)k_chase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_chase, dim3(1), dim3(256), lds, st, a);
+    GqzChainSet s = {};
+    s.H = a.H; s.T = a.T; s.ld = a.ld; s.nch = 1;
+    s.U = a.U; s.V = a.V; s.ustride = 0;
+    s.sr = a.sr; s.si = a.si; s.sb = a.sb; s.smlnum = a.smlnum;
+    s.p[0] = a.p; s.w[0] = a.w; s.nb[0] = a.nb; s.t0[0] = a.t0; s.steps[0] = a.steps;
+    s.intro[0] = a.intro; s.last[0] = a.last; s.soff[0] = 0;
+    gqz_chase_set(st, s);
 }

@@ -22,17 +22,26 @@ static __device__ __forceinline__ dbl4 mfma16(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// workgroup -> (job, panel): jobs are laid out back to back in blockIdx.x
+static __device__ __forceinline__ int slab_job(const GqzSlabJobs &J, int &bx) {
+    int lo = 0, hi = J.njobs - 1;
+    while (lo < hi) {                         // last job with blk0 <= bx
+        int mid = (lo + hi + 1) >> 1;
+        if (J.blk0[mid] <= bx) lo = mid; else hi = mid - 1;
+    }
+    bx -= J.blk0[lo];
+    return lo;
+}
+
 // C[r0 .. r0+RB) x [0,k) <- C * F ;  panel in LDS with leading dimension RB.
-// Several slabs (H above the window, T above, Q, Z) share one launch: blockIdx.x runs over the
-// concatenated row panels of up to four jobs.
 template <int RB>
-__global__ void __launch_bounds__(256) k_slab_right(GqzSlabJobs J, int k) {
+__global__ void __launch_bounds__(256) k_slab_right(GqzSlabJobs J) {
     extern __shared__ double sP[];
     const int tid = threadIdx.x;
-    int bx = blockIdx.x, jb = 0;
-    while (jb < J.njobs - 1 && bx >= J.nblk[jb]) { bx -= J.nblk[jb]; ++jb; }
+    int bx = blockIdx.x;
+    const int jb = slab_job(J, bx);
     double *__restrict__ C = J.C[jb];
-    const int ldc = J.ldc[jb], m = J.m[jb];
+    const int ldc = J.ldc[jb], m = J.m[jb], k = J.k[jb];
     const double *__restrict__ V = J.F[jb];
     const int ldv = J.ldf[jb];
     const long r0 = (long)bx * RB;
@@ -76,21 +85,21 @@ __global__ void __launch_bounds__(256) k_slab_right(GqzSlabJobs J, int k) {
     }
 }
 
-// C[0,k) x [c0 .. c0+CB) <- F^T * C ; panel in LDS with leading dimension k + 1
+// C[0,k) x [c0 .. c0+CB) <- F^T * C ; panel in LDS with leading dimension kmax + 1
 template <int CB>
-__global__ void __launch_bounds__(256) k_slab_left(GqzSlabJobs J, int k) {
+__global__ void __launch_bounds__(256) k_slab_left(GqzSlabJobs J, int kmax) {
     extern __shared__ double sP[];
     const int tid = threadIdx.x;
-    int bx = blockIdx.x, jb = 0;
-    while (jb < J.njobs - 1 && bx >= J.nblk[jb]) { bx -= J.nblk[jb]; ++jb; }
+    int bx = blockIdx.x;
+    const int jb = slab_job(J, bx);
     double *__restrict__ C = J.C[jb];
-    const int ldc = J.ldc[jb], n = J.m[jb];
+    const int ldc = J.ldc[jb], n = J.m[jb], k = J.k[jb];
     const double *__restrict__ U = J.F[jb];
     const int ldu = J.ldf[jb];
     const long c0 = (long)bx * CB;
     if (c0 >= n) return;
     const int cols = (int)min((long)CB, (long)n - c0);
-    const int kp = k + 1;
+    const int kp = kmax + 1;
     for (int idx = tid; idx < k * CB; idx += 256) {
         int i = idx % k, j = idx / k;
         sP[i + kp * j] = (j < cols) ? C[i + (c0 + j) * (long)ldc] : 0.0;
@@ -130,33 +139,59 @@ __global__ void __launch_bounds__(256) k_slab_left(GqzSlabJobs J, int k) {
     }
 }
 
-template <int RB>
-static void launch_right(hipStream_t st, GqzSlabJobs J, int k) {
-    int tot = 0;
-    for (int q = 0; q < J.njobs; ++q) { J.nblk[q] = (J.m[q] + RB - 1) / RB; tot += J.nblk[q]; }
-    if (tot == 0) return;
-    size_t lds = sizeof(double) * RB * (size_t)k;
-    hipFuncSetAttribute((const void *)k_slab_right<RB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_slab_right<RB>, dim3(tot), dim3(256), lds, st, J, k);
-}
-template <int CB>
-static void launch_left(hipStream_t st, GqzSlabJobs J, int k) {
-    int tot = 0;
-    for (int q = 0; q < J.njobs; ++q) { J.nblk[q] = (J.m[q] + CB - 1) / CB; tot += J.nblk[q]; }
-    if (tot == 0) return;
-    size_t lds = sizeof(double) * (size_t)(k + 1) * CB;
-    hipFuncSetAttribute((const void *)k_slab_left<CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_slab_left<CB>, dim3(tot), dim3(256), lds, st, J, k);
+bool gqz_slab_add(GqzSlabJobs &J, double *C, int ldc, int m, const double *F, int ldf, int k) {
+    if (m <= 0 || k <= 0) return true;
+    if (J.njobs >= GQZ_SLAB_MAXJOBS) return false;
+    const int q = J.njobs++;
+    J.C[q] = C; J.ldc[q] = ldc; J.m[q] = m; J.F[q] = F; J.ldf[q] = ldf; J.k[q] = k;
+    return true;
 }
 
-void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs J, int k) {
-    if (k <= 0) return;
+static int slab_kmax(const GqzSlabJobs &J) {
+    int km = 0;
+    for (int q = 0; q < J.njobs; ++q) km = max(km, J.k[q]);
+    return km;
+}
+
+template <int RB>
+static void launch_right(hipStream_t st, GqzSlabJobs &J, int kmax) {
+    int tot = 0;
+    for (int q = 0; q < J.njobs; ++q) { J.blk0[q] = tot; tot += (J.m[q] + RB - 1) / RB; }
+    J.blk0[J.njobs] = tot;
+    if (tot == 0) return;
+    size_t lds = sizeof(double) * RB * (size_t)kmax;
+    static size_t lds_set = 0;
+    if (lds > lds_set) {
+        hipFuncSetAttribute((const void *)k_slab_right<RB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(k_slab_right<RB>, dim3(tot), dim3(256), lds, st, J);
+}
+template <int CB>
+static void launch_left(hipStream_t st, GqzSlabJobs &J, int kmax) {
+    int tot = 0;
+    for (int q = 0; q < J.njobs; ++q) { J.blk0[q] = tot; tot += (J.m[q] + CB - 1) / CB; }
+    J.blk0[J.njobs] = tot;
+    if (tot == 0) return;
+    size_t lds = sizeof(double) * (size_t)(kmax + 1) * CB;
+    static size_t lds_set = 0;
+    if (lds > lds_set) {
+        hipFuncSetAttribute((const void *)k_slab_left<CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(k_slab_left<CB>, dim3(tot), dim3(256), lds, st, J, kmax);
+}
+
+void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs &J) {
+    if (J.njobs <= 0) return;
+    const int k = slab_kmax(J);
     if (k <= 128) launch_right<64>(st, J, k);
     else if (k <= 256) launch_right<32>(st, J, k);
     else launch_right<16>(st, J, k);
 }
-void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs J, int k) {
-    if (k <= 0) return;
+void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs &J) {
+    if (J.njobs <= 0) return;
+    const int k = slab_kmax(J);
     if (k <= 128) launch_left<64>(st, J, k);
     else if (k <= 256) launch_left<32>(st, J, k);
     else launch_left<16>(st, J, k);
@@ -164,18 +199,18 @@ void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs J, int k) {
 void gqz_slab_right(hipStream_t st, double *C, int ldc, int m, int k, const double *V, int ldv,
                     double *ws) {
     (void)ws;
-    if (m <= 0 || k <= 0) return;
-    GqzSlabJobs J = {};
-    J.njobs = 1; J.C[0] = C; J.ldc[0] = ldc; J.m[0] = m; J.F[0] = V; J.ldf[0] = ldv;
-    gqz_slab_right_multi(st, J, k);
+    GqzSlabJobs J;
+    J.njobs = 0;
+    gqz_slab_add(J, C, ldc, m, V, ldv, k);
+    gqz_slab_right_multi(st, J);
 }
 void gqz_slab_left(hipStream_t st, double *C, int ldc, int k, int n, const double *U, int ldu,
                    double *ws) {
     (void)ws;
-    if (n <= 0 || k <= 0) return;
-    GqzSlabJobs J = {};
-    J.njobs = 1; J.C[0] = C; J.ldc[0] = ldc; J.m[0] = n; J.F[0] = U; J.ldf[0] = ldu;
-    gqz_slab_left_multi(st, J, k);
+    GqzSlabJobs J;
+    J.njobs = 0;
+    gqz_slab_add(J, C, ldc, n, U, ldu, k);
+    gqz_slab_left_multi(st, J);
 }
 
 // ---- general out-of-place GEMM: C = op(A) op(B), 64x64 tiles, k-steps of 16 via LDS ---------

@@ -119,9 +119,17 @@ struct KTimer {
     struct Pend { int cls; hipEvent_t e0, e1; double flops, bytes; };
     std::vector<Pend> pend;
     std::vector<hipEvent_t> pool;
-    long launches[GQZ_KC_COUNT] = {};
-    double ms[GQZ_KC_COUNT] = {}, flops[GQZ_KC_COUNT] = {}, bytes[GQZ_KC_COUNT] = {};
+    // classes 0..GQZ_KC_COUNT-1: top level; +GQZ_KC_COUNT: inside AED windows (nested levels)
+    long launches[2 * GQZ_KC_COUNT] = {};
+    double ms[2 * GQZ_KC_COUNT] = {}, flops[2 * GQZ_KC_COUNT] = {}, bytes[2 * GQZ_KC_COUNT] = {};
+    int level = 0;
 } KT;
+
+struct KtLevel {            // scoped recursion level for the timing classes
+    int saved;
+    explicit KtLevel(int l) : saved(KT.level) { KT.level = l; }
+    ~KtLevel() { KT.level = saved; }
+};
 
 hipEvent_t kt_event() {
     if (!KT.pool.empty()) { hipEvent_t e = KT.pool.back(); KT.pool.pop_back(); return e; }
@@ -133,7 +141,7 @@ hipEvent_t kt_event() {
 // returns a slot (or -1 when the class is not timed); the matching kt_end records the stop event
 int kt_begin(int cls) {
     if (!(KT.mask & (1u << cls))) return -1;
-    KTimer::Pend p = { cls, kt_event(), kt_event(), 0.0, 0.0 };
+    KTimer::Pend p = { cls + (KT.level > 0 ? GQZ_KC_COUNT : 0), kt_event(), kt_event(), 0.0, 0.0 };
     HIPCHK(hipEventRecord(p.e0, L.st));
     KT.pend.push_back(p);
     return (int)KT.pend.size() - 1;
@@ -165,46 +173,59 @@ void sync_read(void *dst, const void *src, size_t bytes) {
 
 // algorithmic work of one multi-slab launch: 2 m k^2 flops per slab; each slab read and written
 // once (16 m k bytes) plus its k x k factor (8 k^2 bytes)
-void slab_work(const GqzSlabJobs &J, int k, double &flops, double &bytes) {
+void slab_work(const GqzSlabJobs &J, double &flops, double &bytes) {
     flops = 0.0; bytes = 0.0;
     for (int q = 0; q < J.njobs; ++q) {
-        flops += 2.0 * J.m[q] * (double)k * k;
-        bytes += 16.0 * J.m[q] * (double)k + 8.0 * (double)k * k;
+        const double k = J.k[q];
+        flops += 2.0 * J.m[q] * k * k;
+        bytes += 16.0 * J.m[q] * k + 8.0 * k * k;
     }
 }
 
-// ---- PDHGEQZ6 role: apply the window's U, V outside the window [p, p+w) -------------------
+// ---- PDHGEQZ6 role: apply the windows' U, V outside the windows ----------------------------
+// Windows [p_i, p_i + w_i) of one round are disjoint.  All left updates (rows of a window,
+// columns to its right) are launched before all right updates (columns of a window, rows above,
+// Q, Z): the only blocks both touch are (rows of a lower window) x (columns of an upper one),
+// where the two products commute.
+struct Win { int p, w; const double *U, *V; };
+
+void slab_launch(Run &R, GqzSlabJobs &J, bool left) {
+    if (J.njobs == 0) return;
+    double fl, by;
+    slab_work(J, fl, by);
+    int ks = kt_begin(GQZ_KC_SLAB);
+    if (left) gqz_slab_left_multi(L.st, J);
+    else gqz_slab_right_multi(L.st, J);
+    kt_end(ks, fl, by);
+    R.st.slab_launches++;
+    J.njobs = 0;
+}
+
+void apply_windows(Run &R, const View &v, const Win *ws, int nw) {
+    GqzSlabJobs J;
+    J.njobs = 0;
+    for (int i = 0; i < nw; ++i) {
+        const Win &x = ws[i];
+        const int ncol = v.n - x.p - x.w;
+        if (J.njobs + 2 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, true);
+        gqz_slab_add(J, v.H + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, ncol, x.U, x.w, x.w);
+        gqz_slab_add(J, v.T + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, ncol, x.U, x.w, x.w);
+    }
+    slab_launch(R, J, true);
+    for (int i = 0; i < nw; ++i) {
+        const Win &x = ws[i];
+        if (J.njobs + 4 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, false);
+        gqz_slab_add(J, v.H + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+        gqz_slab_add(J, v.T + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+        if (v.Q) gqz_slab_add(J, v.Q + (size_t)x.p * v.ldq, v.ldq, v.nq, x.U, x.w, x.w);
+        if (v.Z) gqz_slab_add(J, v.Z + (size_t)x.p * v.ldq, v.ldq, v.nq, x.V, x.w, x.w);
+    }
+    slab_launch(R, J, false);
+}
+
 void apply_window(Run &R, const View &v, int p, int w, const double *U, const double *V) {
-    GqzSlabJobs Jl = {};
-    int ncol = v.n - p - w;
-    if (ncol > 0) {
-        Jl.njobs = 2;
-        Jl.C[0] = v.H + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[0] = v.ld; Jl.m[0] = ncol; Jl.F[0] = U; Jl.ldf[0] = w;
-        Jl.C[1] = v.T + (size_t)p + (size_t)(p + w) * v.ld; Jl.ldc[1] = v.ld; Jl.m[1] = ncol; Jl.F[1] = U; Jl.ldf[1] = w;
-        double fl, by;
-        slab_work(Jl, w, fl, by);
-        int ks = kt_begin(GQZ_KC_SLAB);
-        gqz_slab_left_multi(L.st, Jl, w);
-        kt_end(ks, fl, by);
-        R.st.slab_launches++;
-    }
-    GqzSlabJobs Jr = {};
-    int q = 0;
-    if (p > 0) {
-        Jr.C[q] = v.H + (size_t)p * v.ld; Jr.ldc[q] = v.ld; Jr.m[q] = p; Jr.F[q] = V; Jr.ldf[q] = w; ++q;
-        Jr.C[q] = v.T + (size_t)p * v.ld; Jr.ldc[q] = v.ld; Jr.m[q] = p; Jr.F[q] = V; Jr.ldf[q] = w; ++q;
-    }
-    if (v.Q) { Jr.C[q] = v.Q + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = U; Jr.ldf[q] = w; ++q; }
-    if (v.Z) { Jr.C[q] = v.Z + (size_t)p * v.ldq; Jr.ldc[q] = v.ldq; Jr.m[q] = v.nq; Jr.F[q] = V; Jr.ldf[q] = w; ++q; }
-    Jr.njobs = q;
-    if (q > 0) {
-        double fl, by;
-        slab_work(Jr, w, fl, by);
-        int ks = kt_begin(GQZ_KC_SLAB);
-        gqz_slab_right_multi(L.st, Jr, w);
-        kt_end(ks, fl, by);
-        R.st.slab_launches++;
-    }
+    Win x = { p, w, U, V };
+    apply_windows(R, v, &x, 1);
 }
 
 // ---- small active block: LDS-resident base QZ, then the off-block updates ------------------
@@ -239,7 +260,7 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     size_t mk = L.ws.mark();
     size_t jj = (size_t)jw * jw;
     double *Hw = L.ws.alloc(jj), *Tw = L.ws.alloc(jj), *U = L.ws.alloc(jj), *V = L.ws.alloc(jj);
-    double *weig = L.ws.alloc(3 * (size_t)jw), *scr = L.ws.alloc(jw);
+    double *weig = L.ws.alloc(3 * (size_t)jw), *scr = L.ws.alloc(4 * (size_t)jw);
     gqz_launch_window_in(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, V);
     View W;
     W.H = Hw; W.T = Tw; W.Q = U; W.Z = V; W.ld = jw; W.ldq = jw; W.n = jw; W.nq = jw;
@@ -269,23 +290,32 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     int kr = kt_begin(GQZ_KC_REORDER);
     gqz_launch_aed_reorder(L.st, Hw, Tw, U, V, jw, sptr, W.smlnum, pscr, L.d_out + 16);
     kt_end(kr, 0.0, 32.0 * jj);
-    int kp = kt_begin(GQZ_KC_SPIKE);
-    gqz_launch_aed_spike(L.st, Hw, Tw, U, V, jw, sptr, W.ar, W.ai, W.be, scr, L.d_out + 16);
-    kt_end(kp, 0.0, 32.0 * jj);
+    gqz_launch_aed_eigs(L.st, Hw, Tw, jw, W.ar, W.ai, W.be);
     HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 4 * sizeof(int), hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipStreamSynchronize(L.st));
+    if (!KT.pend.empty()) kt_drain();
     int kbc = L.h_out[16];
     R.st.swaps += L.h_out[17];
     R.st.swap_fail += L.h_out[18] ? 1 : 0;
     int spike = L.h_out[19];
-    if (kbc > 0 && spike) {
-        int kg = kt_begin(GQZ_KC_HT);
-        gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
-        kt_end(kg, 0.0, 32.0 * jj);
+    // nothing deflated: the window is left as it was (PDHGEQZ3.f:840, :1555); its eigenvalues are
+    // still returned as shifts
+    const bool restore = (kbc < jw) || !spike;
+    if (restore) {
+        if (kbc > 0 && spike) {
+            int kp = kt_begin(GQZ_KC_SPIKE);
+            gqz_launch_aed_spike(L.st, Hw, Tw, U, V, jw, sptr, scr, L.d_out + 16);
+            kt_end(kp, 0.0, 32.0 * jj);
+        }
+        if (kbc > 2 && spike) {
+            int kg = kt_begin(GQZ_KC_HT);
+            gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+            kt_end(kg, 0.0, 32.0 * jj);
+        }
+        gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? 1 : 0);
+        apply_window(R, v, kwtop, jw, U, V);
     }
-    gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? 1 : 0);
-    apply_window(R, v, kwtop, jw, U, V);
     sh.assign(3 * (size_t)kbc, 0.0);
     for (int i = 0; i < kbc; ++i) {
         sh[i] = L.h_buf[i];
@@ -299,38 +329,73 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     return jw - kbc;
 }
 
-// ---- one chain of nb bulges from ktop to kbot (PDHGEQZ5 with NUMWIN = 1) -------------------
-void chase_chain(Run &R, const View &v, int ktop, int kbot, int nb, const double *sr, const double *si,
-                 const double *sb) {
+// ---- the chains of one sweep, chased concurrently (PDHGEQZ5 with several windows) -----------
+// Chain c (shift pairs soff..) follows chain c-1 down the diagonal.  In every round each chain
+// that can move chases its bulges through one window (one workgroup per chain, one launch),
+// then all windows' U, V are applied.  A chain's window stays above the window (or, when that
+// chain waits, above the bulges) of the chain ahead of it, so the windows of a round are
+// disjoint.  Each chain sees exactly the window sequence of a lone chain (intro window at ktop,
+// then windows starting where the previous one stopped), so the result does not depend on how
+// many chains share a round.
+struct Chain { int soff, nb, gtop; bool started, done; };
+
+void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> &ch, const double *d_sr,
+                  const double *d_si, const double *d_sb) {
     const int WMAX = gqz_chase_max_window();
+    const int nslot = GQZ_MAXCH;
     size_t mk = L.ws.mark();
-    double *U = L.ws.alloc((size_t)WMAX * WMAX), *V = L.ws.alloc((size_t)WMAX * WMAX);
-    GqzChainArgs a = {};
-    a.H = v.H; a.T = v.T; a.ld = v.ld; a.nb = nb; a.U = U; a.V = V; a.sr = sr; a.si = si; a.sb = sb;
-    a.smlnum = v.smlnum;
-    int p = ktop;
-    int w = std::min(WMAX, kbot - ktop + 1);
-    int last = (p + w - 1 == kbot);
-    a.p = p; a.w = w; a.intro = 1; a.t0 = -1; a.last = last;
-    a.steps = last ? w : (w - 3 * nb - 1);
-    int kc = kt_begin(GQZ_KC_CHASE);
-    gqz_chase_window(L.st, a);
-    kt_end(kc, 0.0, 48.0 * w * w);
-    apply_window(R, v, p, w, U, V);
-    R.st.chase_windows++;
-    int gtop = p + a.steps;
-    while (!last) {
-        p = gtop;
-        w = std::min(WMAX, kbot - p + 1);
-        last = (p + w - 1 == kbot);
-        a.p = p; a.w = w; a.intro = 0; a.t0 = 0; a.last = last;
-        a.steps = last ? w : (w - 3 * nb - 1);
-        int kc2 = kt_begin(GQZ_KC_CHASE);
-        gqz_chase_window(L.st, a);
-        kt_end(kc2, 0.0, 48.0 * w * w);
-        apply_window(R, v, p, w, U, V);
-        R.st.chase_windows++;
-        gtop = p + a.steps;
+    double *U = L.ws.alloc((size_t)nslot * WMAX * WMAX), *V = L.ws.alloc((size_t)nslot * WMAX * WMAX);
+    const int nch = (int)ch.size();
+    int first = 0;                       // oldest chain not yet done
+    while (first < nch) {
+        GqzChainSet S = {};
+        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = U; S.V = V; S.ustride = WMAX * WMAX;
+        S.sr = d_sr; S.si = d_si; S.sb = d_sb; S.smlnum = v.smlnum;
+        Win ws[GQZ_MAXCH];
+        int slot_chain[GQZ_MAXCH];
+        int nw = 0;
+        // rows from which the chain ahead (already scheduled this round) forbids access
+        int limit = kbot + 1;
+        int active = 0;
+        for (int c = first; c < nch && nw < nslot; ++c) {
+            Chain &x = ch[c];
+            if (x.done) continue;
+            if (!x.started && active >= nslot) break;
+            int p = x.started ? x.gtop : ktop;
+            int w = std::min(WMAX, kbot - p + 1);
+            if (p + w > limit) {
+                // wait: the chain ahead still occupies rows below; nothing behind may pass
+                limit = x.started ? x.gtop : ktop;
+                if (!x.started) break;
+                ++active;
+                continue;
+            }
+            const int last = (p + w - 1 == kbot);
+            const int q = nw++;
+            S.p[q] = p; S.w[q] = w; S.nb[q] = x.nb; S.soff[q] = x.soff;
+            S.intro[q] = x.started ? 0 : 1; S.t0[q] = x.started ? 0 : -1; S.last[q] = last;
+            S.steps[q] = last ? w : (w - 3 * x.nb - 1);
+            ws[q] = Win{ p, w, U + (size_t)q * WMAX * WMAX, V + (size_t)q * WMAX * WMAX };
+            slot_chain[q] = c;
+            limit = p;
+            ++active;
+        }
+        if (nw == 0) throw GqzError("chain scheduler made no progress");
+        S.nch = nw;
+        double by = 0.0;
+        for (int q = 0; q < nw; ++q) by += 48.0 * S.w[q] * S.w[q];
+        int kc = kt_begin(GQZ_KC_CHASE);
+        gqz_chase_set(L.st, S);
+        kt_end(kc, 0.0, by);
+        apply_windows(R, v, ws, nw);
+        R.st.chase_windows += nw;
+        for (int q = 0; q < nw; ++q) {
+            Chain &x = ch[slot_chain[q]];
+            x.started = true;
+            x.gtop = S.p[q] + S.steps[q];
+            if (S.last[q]) x.done = true;
+        }
+        while (first < nch && ch[first].done) ++first;
     }
     L.ws.release(mk);
 }
@@ -397,12 +462,14 @@ void sweep(Run &R, const View &v, int ktop, int kbot, std::vector<double> &sh, i
     int chain = R.prm.chain > 0 ? R.prm.chain : 20;
     chain = std::max(2, std::min(chain, 40));
     chain -= chain % 2;
+    std::vector<Chain> chains;
     for (int off = 0; off < ns; off += chain) {
         int cnt = std::min(chain, ns - off);
         cnt -= cnt % 2;
         if (cnt < 2) break;
-        chase_chain(R, v, ktop, kbot, cnt / 2, d_sh + off, d_sh + ns + off, d_sh + 2 * ns + off);
+        chains.push_back(Chain{ off, cnt / 2, ktop, false, false });
     }
+    chase_chains(R, v, ktop, kbot, chains, d_sh, d_sh + ns, d_sh + 2 * ns);
     // the staging buffer is reused by the next readback: make sure the upload has completed
     HIPCHK(hipStreamSynchronize(L.st));
     L.ws.release(mk);
@@ -436,6 +503,7 @@ void features(Run &R, int nh, float f[12]) {
 }
 
 int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
+    KtLevel klev(level);
     int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
     int nh0 = ihi - ilo + 1;
     if (nh0 <= 0) return 0;
@@ -610,18 +678,20 @@ int gqz_ktime_enable(unsigned mask) {
         kt_drain();
     }
     KT.mask = mask;
-    for (int c = 0; c < GQZ_KC_COUNT; ++c) { KT.launches[c] = 0; KT.ms[c] = KT.flops[c] = KT.bytes[c] = 0.0; }
+    for (int c = 0; c < 2 * GQZ_KC_COUNT; ++c) { KT.launches[c] = 0; KT.ms[c] = KT.flops[c] = KT.bytes[c] = 0.0; }
     return 0;
 }
 
 int gqz_ktime_get(gqz_ktime *out, int cap) {
-    static const char *names[GQZ_KC_COUNT] = { "slab_update", "chase_window", "small_qz", "aed_reorder",
-                                               "aed_spike", "ht_givens", "inf_deflation" };
+    static const char *names[2 * GQZ_KC_COUNT] = {
+        "slab_update", "chase_window", "small_qz", "aed_reorder", "aed_spike", "ht_givens", "inf_deflation",
+        "slab_update@aed", "chase_window@aed", "small_qz@aed", "aed_reorder@aed", "aed_spike@aed",
+        "ht_givens@aed", "inf_deflation@aed" };
     if (L.dev >= 0) {
         if (hipStreamSynchronize(L.st) != hipSuccess) return -1;
         kt_drain();
     }
-    int k = std::min(cap, (int)GQZ_KC_COUNT);
+    int k = std::min(cap, (int)(2 * GQZ_KC_COUNT));
     for (int c = 0; c < k; ++c) {
         out[c].name = names[c]; out[c].launches = KT.launches[c]; out[c].ms = KT.ms[c];
         out[c].flops = KT.flops[c]; out[c].bytes = KT.bytes[c];

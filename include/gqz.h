@@ -140,7 +140,9 @@ typedef struct {
     double ms, flops, bytes;
 } gqz_ktime;
 int gqz_ktime_enable(unsigned mask);          /* resets the totals */
-int gqz_ktime_get(gqz_ktime *out, int cap);   /* synchronises; returns the number of classes */
+/* synchronises; returns the number of entries (2 * GQZ_KC_COUNT: each class at the top level, then
+ * inside AED windows, named "<class>@aed") */
+int gqz_ktime_get(gqz_ktime *out, int cap);
 
 /* ---- batched pencils (no counterpart in the reference) ---------------------------------------
  * count pencils of orders n[i] stored back to back (column-major, ld = n[i]) at offsets off[i]
