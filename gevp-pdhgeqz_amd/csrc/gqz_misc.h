@@ -24,6 +24,13 @@ void gqz_launch_sub(hipStream_t st, double *R, int ldr, const double *S, int lds
 // AED window post-processing (k_aed.hip); pscr holds (jw/2 + 1) * 65 doubles
 void gqz_launch_aed_reorder(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
                             const double *sptr, double smlnum, double *pscr, int *out);
+// whole AED of a window of order jw <= gqz_aed_small_max() in one LDS-resident kernel; out: 5 ints
+// (kbc, swaps, reorder failed, spike present, INFO of the window QZ); pscr: (jw/2+1)*65 doubles,
+// scr: 4 jw doubles
+int gqz_aed_small_max();
+void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwtop, int jw, int has_spike,
+                          double ttol, double smlnum, double *U, double *V, double *war, double *wai, double *wb,
+                          double *pscr, double *scr, int *out);
 void gqz_launch_aed_eigs(hipStream_t st, const double *Hw, const double *Tw, int jw, double *war, double *wai,
                          double *wb);
 // spike reflector + RQ update of T (scr: 4 * jw doubles); reads kbc from out[0]

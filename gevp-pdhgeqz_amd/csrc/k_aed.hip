@@ -12,7 +12,10 @@
 //   while the number of dependent steps drops from O(jw^2) to O(jw).
 // k_aed_spike: eigenvalues of the window blocks, the spike reflector (qzearly.f:369-386) and
 //   the RQ factorisation of T (qzearly.f:387-392).
-#include "gqz_wg.h"
+#include <hip/hip_cooperative_groups.h>
+#include <algorithm>
+
+#include "gqz_ds.h"
 #include "gqz_swap.h"
 #include "gqz_misc.h"
 
@@ -23,15 +26,15 @@ enum { ST_U = 0, ST_X = 1, ST_A = 2, ST_D = 3 };
 #define PREC 65
 
 template <int N1, int N2>
-__device__ __forceinline__ void pair_swap(const double *Hw, const double *Tw, int jw, int j1, double *rec) {
+__device__ __forceinline__ void pair_swap(const double *Hw, const double *Tw, int ld, int j1, double *rec) {
     constexpr int M = N1 + N2;
     double S[M * M], T[M * M], Qa[M * M], Za[M * M];
 #pragma unroll
     for (int j = 0; j < M; ++j)
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            S[i + M * j] = Hw[GQZ_IX(j1 + i, j1 + j, jw)];
-            T[i + M * j] = Tw[GQZ_IX(j1 + i, j1 + j, jw)];
+            S[i + M * j] = Hw[GQZ_IX(j1 + i, j1 + j, ld)];
+            T[i + M * j] = Tw[GQZ_IX(j1 + i, j1 + j, ld)];
         }
     int f = r_tgex2<N1, N2>(S, T, Qa, Za);
     rec[64] = f ? 1.0 : 0.0;
@@ -40,42 +43,50 @@ __device__ __forceinline__ void pair_swap(const double *Hw, const double *Tw, in
     for (int q = 0; q < M * M; ++q) { rec[q] = Qa[q]; rec[16 + q] = Za[q]; rec[32 + q] = S[q]; rec[48 + q] = T[q]; }
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, double *Tw, double *U, double *V,
-                                                                int jw, const double *sptr, double smlnum,
-                                                                double *pscr, int *out) {
-    __shared__ unsigned char st[AED_MAXW];
-    __shared__ short tgt[AED_MAXW];
-    __shared__ short pj1[AED_MAXW / 2];
-    __shared__ unsigned char pn1[AED_MAXW / 2], pn2[AED_MAXW / 2];
+// Block structure lives in LDS: bsz[p] = order of the block starting at row p (0 inside a 2x2
+// block), st[p] / tgt[p] = state and target row of that block.  One round:
+//   test (lane 0)  -> pair detection (all lanes, LDS) -> swap factors (one lane per pair)
+//   -> rows of every pair -> columns of every pair -> block-structure update (one lane per pair).
+template <int MAXW>
+__device__ void aed_reorder_body(double *Hw, double *Tw, double *U, double *V, int ld, int jw, double s,
+                                 double smlnum, double *pscr, int *out) {
+    __shared__ unsigned char st[MAXW], bsz[MAXW];
+    __shared__ short tgt[MAXW];
+    __shared__ short pj1[MAXW / 2];
+    __shared__ unsigned char pn1[MAXW / 2], pn2[MAXW / 2];
     __shared__ int s_np, s_kb, s_ilst, s_knt, s_fail, s_swaps;
-    const int tid = threadIdx.x;
-    const double s = sptr ? *sptr : 0.0;
-    for (int i = tid; i < jw; i += blockDim.x) { st[i] = ST_U; tgt[i] = 0; }
-    if (tid == 0) { s_kb = jw - 1; s_ilst = 0; s_knt = 0; s_fail = 0; s_swaps = 0; }
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < jw; i += nt) {
+        st[i] = ST_U; tgt[i] = 0;
+        const bool lower = (i > 0) && Hw[GQZ_IX(i, i - 1, ld)] != 0.0;
+        const bool upper = (i + 1 < jw) && Hw[GQZ_IX(i + 1, i, ld)] != 0.0;
+        bsz[i] = lower ? 0 : (upper ? 2 : 1);
+    }
+    if (tid == 0) { s_kb = jw - 1; s_ilst = 0; s_knt = 0; s_fail = 0; s_swaps = 0; s_np = 0; }
     __syncthreads();
     for (int round = 0; round < 4 * jw + 8; ++round) {
         if (tid == 0) {
             // ---- test phase: examine blocks at the bottom while they are unexamined ----
             while (s_kb >= 0 && s_knt < jw) {
-                int kb = s_kb;
-                int b0 = (kb > 0 && Hw[GQZ_IX(kb, kb - 1, jw)] != 0.0) ? kb - 1 : kb;
+                const int kb = s_kb;
+                const int b0 = (bsz[kb] == 0) ? kb - 1 : kb;
                 if (st[b0] != ST_U) break;
-                int bulge = (b0 != kb);
+                const int bulge = (b0 != kb);
                 bool dfl;
                 if (!bulge) {
                     s_knt += 1;
-                    double tmp = fabs(s) + fabs(Hw[GQZ_IX(kb, kb, jw)]);
-                    dfl = fabs(s * U[GQZ_IX(0, kb, jw)]) <= fmax(GQZ_ULP * tmp, smlnum);
+                    double tmp = fabs(s) + fabs(Hw[GQZ_IX(kb, kb, ld)]);
+                    dfl = fabs(s * U[GQZ_IX(0, kb, ld)]) <= fmax(GQZ_ULP * tmp, smlnum);
                 } else {
                     s_knt += 2;
-                    double h00 = Hw[GQZ_IX(kb - 1, kb - 1, jw)], h10 = Hw[GQZ_IX(kb, kb - 1, jw)];
-                    double h01 = Hw[GQZ_IX(kb - 1, kb, jw)], h11 = Hw[GQZ_IX(kb, kb, jw)];
+                    double h00 = Hw[GQZ_IX(kb - 1, kb - 1, ld)], h10 = Hw[GQZ_IX(kb, kb - 1, ld)];
+                    double h01 = Hw[GQZ_IX(kb - 1, kb, ld)], h11 = Hw[GQZ_IX(kb, kb, ld)];
                     double rtdet;
                     if (h00 == 0.0 && h10 == 0.0) rtdet = 0.0;
                     else if (fabs(h00) >= fabs(h10)) rtdet = sqrt(fabs(h00)) * sqrt(fabs(h11 - (h10 / h00) * h01));
                     else rtdet = sqrt(fabs(h10)) * sqrt(fabs(h01 - (h00 / h10) * h11));
                     double tmp = fabs(s) + rtdet;
-                    dfl = fmax(fabs(s * U[GQZ_IX(0, kb, jw)]), fabs(s * U[GQZ_IX(0, kb - 1, jw)])) <=
+                    dfl = fmax(fabs(s * U[GQZ_IX(0, kb, ld)]), fabs(s * U[GQZ_IX(0, kb - 1, ld)])) <=
                           fmax(GQZ_ULP * tmp, smlnum);
                 }
                 if (dfl) { st[b0] = ST_D; s_kb = b0 - 1; }
@@ -86,39 +97,37 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
                     break;
                 }
             }
-            // ---- pair phase: (unexamined block above, moving block below) ----
-            int np = 0;
-            int pst = -1, psz = 0, pstate = ST_A;
-            for (int p = 0; p <= s_kb;) {
-                int sz = (p + 1 < jw && Hw[GQZ_IX(p + 1, p, jw)] != 0.0) ? 2 : 1;
-                int stt = st[p];
-                if (stt == ST_X && p > tgt[p] && pstate == ST_U && pst >= 0) {
-                    pj1[np] = (short)pst; pn1[np] = (unsigned char)psz; pn2[np] = (unsigned char)sz; ++np;
-                    // the pair consumes both blocks; the next block has no usable predecessor
-                    pst = -1; pstate = ST_A; p += sz;
-                    continue;
-                }
-                pst = p; psz = sz; pstate = stt;
-                p += sz;
-            }
-            s_np = np;
+            s_np = 0;
+        }
+        __syncthreads();
+        // ---- pair detection: a moving block whose upper neighbour is unexamined ----
+        const int kbnow = s_kb;
+        for (int p = tid; p <= kbnow; p += nt) {
+            if (bsz[p] == 0 || st[p] != ST_X || p <= tgt[p] || p == 0) continue;
+            const int q = (bsz[p - 1] != 0) ? p - 1 : p - 2;
+            if (q < 0 || st[q] != ST_U) continue;
+            const int k = atomicAdd(&s_np, 1);
+            pj1[k] = (short)q; pn1[k] = bsz[q]; pn2[k] = bsz[p];
         }
         __syncthreads();
         const int np = s_np;
         if (np == 0) break;
         // ---- compute the swaps (one lane per pair, register resident) ----
-        for (int q = tid; q < np; q += blockDim.x) {
+        for (int q = tid; q < np; q += nt) {
             const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
             double *rec = pscr + (size_t)PREC * q;
-            if (n1 == 1 && n2 == 1) pair_swap<1, 1>(Hw, Tw, jw, j1, rec);
-            else if (n1 == 1 && n2 == 2) pair_swap<1, 2>(Hw, Tw, jw, j1, rec);
-            else if (n1 == 2 && n2 == 1) pair_swap<2, 1>(Hw, Tw, jw, j1, rec);
-            else pair_swap<2, 2>(Hw, Tw, jw, j1, rec);
+            if (n1 == 1 && n2 == 1) pair_swap<1, 1>(Hw, Tw, ld, j1, rec);
+            else if (n1 == 1 && n2 == 2) pair_swap<1, 2>(Hw, Tw, ld, j1, rec);
+            else if (n1 == 2 && n2 == 1) pair_swap<2, 1>(Hw, Tw, ld, j1, rec);
+            else pair_swap<2, 2>(Hw, Tw, ld, j1, rec);
         }
         __syncthreads();
-        // ---- rows of every pair (cols right of its block) and U columns ----
-        for (size_t idx = tid; idx < (size_t)np * 3 * jw; idx += blockDim.x) {
-            const int q = (int)(idx / (3 * jw)), r = (int)(idx % (3 * jw));
+        // ---- rows of every pair (cols right of its block) and U columns; the (pair, item) index
+        //      space is flattened so that all lanes work on independent items ----
+        const int d1 = 3 * jw;
+        const double inv1 = 1.0 / d1;
+        for (int idx = tid; idx < np * d1; idx += nt) {
+            const int q = (int)((idx + 0.5) * inv1), r = idx - q * d1;
             const double *rec = pscr + (size_t)PREC * q;
             if (rec[64] != 0.0) continue;
             const int j1 = pj1[q], m = pn1[q] + pn2[q];
@@ -128,20 +137,22 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
                 double *M = r < jw ? Hw : Tw;
                 const int col = r < jw ? r : r - jw;
                 if (col < j1 + m) continue;
-                for (int i = 0; i < m; ++i) x[i] = M[GQZ_IX(j1 + i, col, jw)];
+                for (int i = 0; i < m; ++i) x[i] = M[GQZ_IX(j1 + i, col, ld)];
                 for (int i = 0; i < m; ++i) { double a = 0; for (int k = 0; k < m; ++k) a += Qa[k + m * i] * x[k]; y[i] = a; }
-                for (int i = 0; i < m; ++i) M[GQZ_IX(j1 + i, col, jw)] = y[i];
+                for (int i = 0; i < m; ++i) M[GQZ_IX(j1 + i, col, ld)] = y[i];
             } else {
                 const int row = r - 2 * jw;
-                for (int j = 0; j < m; ++j) x[j] = U[GQZ_IX(row, j1 + j, jw)];
+                for (int j = 0; j < m; ++j) x[j] = U[GQZ_IX(row, j1 + j, ld)];
                 for (int j = 0; j < m; ++j) { double a = 0; for (int k = 0; k < m; ++k) a += x[k] * Qa[k + m * j]; y[j] = a; }
-                for (int j = 0; j < m; ++j) U[GQZ_IX(row, j1 + j, jw)] = y[j];
+                for (int j = 0; j < m; ++j) U[GQZ_IX(row, j1 + j, ld)] = y[j];
             }
         }
         __syncthreads();
         // ---- columns of every pair (rows above its block), V, and the new diagonal blocks ----
-        for (size_t idx = tid; idx < (size_t)np * (3 * jw + 16); idx += blockDim.x) {
-            const int q = (int)(idx / (3 * jw + 16)), r = (int)(idx % (3 * jw + 16));
+        const int d2 = 3 * jw + 16;
+        const double inv2 = 1.0 / d2;
+        for (int idx = tid; idx < np * d2; idx += nt) {
+            const int q = (int)((idx + 0.5) * inv2), r = idx - q * d2;
             const double *rec = pscr + (size_t)PREC * q;
             if (rec[64] != 0.0) continue;
             const int j1 = pj1[q], m = pn1[q] + pn2[q];
@@ -151,8 +162,8 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
                 const int e = r - 3 * jw;
                 if (e < m * m) {
                     const int i = e % m, j = e / m;
-                    Hw[GQZ_IX(j1 + i, j1 + j, jw)] = rec[32 + e];
-                    Tw[GQZ_IX(j1 + i, j1 + j, jw)] = rec[48 + e];
+                    Hw[GQZ_IX(j1 + i, j1 + j, ld)] = rec[32 + e];
+                    Tw[GQZ_IX(j1 + i, j1 + j, ld)] = rec[48 + e];
                 }
                 continue;
             }
@@ -160,26 +171,35 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
             if (r < jw) { M = Hw; row = r; if (row >= j1) continue; }
             else if (r < 2 * jw) { M = Tw; row = r - jw; if (row >= j1) continue; }
             else { M = V; row = r - 2 * jw; }
-            for (int j = 0; j < m; ++j) x[j] = M[GQZ_IX(row, j1 + j, jw)];
+            for (int j = 0; j < m; ++j) x[j] = M[GQZ_IX(row, j1 + j, ld)];
             for (int j = 0; j < m; ++j) { double a = 0; for (int k = 0; k < m; ++k) a += x[k] * Za[k + m * j]; y[j] = a; }
-            for (int j = 0; j < m; ++j) M[GQZ_IX(row, j1 + j, jw)] = y[j];
+            for (int j = 0; j < m; ++j) M[GQZ_IX(row, j1 + j, ld)] = y[j];
         }
         __syncthreads();
-        // ---- state update ----
-        if (tid == 0) {
-            for (int q = 0; q < np; ++q) {
-                const double *rec = pscr + (size_t)PREC * q;
-                if (rec[64] != 0.0) { s_fail = 1; continue; }
-                const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
-                const int t = tgt[j1 + n1];
-                s_swaps++;
-                // the moving block now starts at j1 (order n2), the passed block at j1 + n2 (n1)
-                bool xsplit = (n2 == 2) && Hw[GQZ_IX(j1 + 1, j1, jw)] == 0.0;
+        // ---- block-structure update (one lane per pair; pairs are disjoint) ----
+        for (int q = tid; q < np; q += nt) {
+            const double *rec = pscr + (size_t)PREC * q;
+            if (rec[64] != 0.0) { s_fail = 1; continue; }
+            const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
+            const int t = tgt[j1 + n1];
+            atomicAdd(&s_swaps, 1);
+            // the moving block now starts at j1 (order n2), the passed block at j1 + n2 (order n1);
+            // a 2x2 block whose eigenvalues turned real after the swap splits into two 1x1 blocks
+            const bool msplit = (n2 == 2) && Hw[GQZ_IX(j1 + 1, j1, ld)] == 0.0;
+            const bool psplit = (n1 == 2) && Hw[GQZ_IX(j1 + n2 + 1, j1 + n2, ld)] == 0.0;
+            if (msplit) {
+                bsz[j1] = 1; bsz[j1 + 1] = 1;
                 st[j1] = (j1 == t) ? ST_A : ST_X; tgt[j1] = (short)t;
-                if (xsplit) { st[j1 + 1] = (j1 + 1 == t + 1) ? ST_A : ST_X; tgt[j1 + 1] = (short)(t + 1); }
-                st[j1 + n2] = ST_U;
-                if (n1 == 2) st[j1 + n2 + 1] = ST_U;
+                st[j1 + 1] = (j1 + 1 == t + 1) ? ST_A : ST_X; tgt[j1 + 1] = (short)(t + 1);
+            } else {
+                bsz[j1] = (unsigned char)n2;
+                if (n2 == 2) bsz[j1 + 1] = 0;
+                st[j1] = (j1 == t) ? ST_A : ST_X; tgt[j1] = (short)t;
             }
+            const int pb = j1 + n2;
+            if (n1 == 2 && !psplit) { bsz[pb] = 2; bsz[pb + 1] = 0; st[pb] = ST_U; }
+            else if (n1 == 2) { bsz[pb] = 1; bsz[pb + 1] = 1; st[pb] = ST_U; st[pb + 1] = ST_U; }
+            else { bsz[pb] = 1; st[pb] = ST_U; }
         }
         __syncthreads();
         if (s_fail) break;
@@ -189,18 +209,236 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_reorder(double *Hw, doub
         if (!s_fail && kbc < s_ilst) kbc = s_ilst;
         out[0] = kbc; out[1] = s_swaps; out[2] = s_fail; out[3] = (s != 0.0);
     }
+    __syncthreads();
+}
+
+extern "C" __global__ void __launch_bounds__(512) k_aed_reorder(double *Hw, double *Tw, double *U, double *V,
+                                                                 int jw, const double *sptr, double smlnum,
+                                                                 double *pscr, int *out) {
+    aed_reorder_body<AED_MAXW>(Hw, Tw, U, V, jw, jw, sptr ? *sptr : 0.0, smlnum, pscr, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_aed_reorder_coop: the same train on a large window with the row/column updates spread over
+// a cooperative grid.  Every workgroup keeps its own copy of the block structure and of every
+// pair's swap factors in LDS and recomputes them identically (same inputs after each grid
+// barrier), so a round needs only two grid barriers: one after the row updates and one after the
+// column updates.  Pairs are numbered by position (a scan, not atomics) so that all workgroups
+// agree on the numbering.  The new diagonal blocks travel through pscr, written by the pair's
+// owner workgroup.
+// ---------------------------------------------------------------------------------------------
+#define COOP_MAXW 1024
+#define COOP_NT 512
+
+extern "C" __global__ void __launch_bounds__(COOP_NT) k_aed_reorder_coop(double *Hw, double *Tw, double *U,
+                                                                         double *V, int jw, const double *sptr,
+                                                                         double smlnum, double *pscr, int *out) {
+    namespace cg = cooperative_groups;
+    __shared__ unsigned char st[COOP_MAXW], bsz[COOP_MAXW];
+    __shared__ short tgt[COOP_MAXW];
+    __shared__ short pj1[COOP_MAXW / 2];
+    __shared__ unsigned char pn1[COOP_MAXW / 2], pn2[COOP_MAXW / 2], pfl[COOP_MAXW / 2];
+    __shared__ double pq[COOP_MAXW / 2][32];          // Qa (m x m, ld m) | Za
+    __shared__ int scan[COOP_NT];
+    __shared__ int s_np, s_kb, s_ilst, s_knt, s_fail, s_swaps;
+    cg::grid_group grid = cg::this_grid();
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int gtid = blockIdx.x * nt + tid, gnt = gridDim.x * nt;
+    const double s = sptr ? *sptr : 0.0;
+    const int ld = jw;
+    for (int i = tid; i < jw; i += nt) {
+        st[i] = ST_U; tgt[i] = 0;
+        const bool lower = (i > 0) && Hw[GQZ_IX(i, i - 1, ld)] != 0.0;
+        const bool upper = (i + 1 < jw) && Hw[GQZ_IX(i + 1, i, ld)] != 0.0;
+        bsz[i] = lower ? 0 : (upper ? 2 : 1);
+    }
+    if (tid == 0) { s_kb = jw - 1; s_ilst = 0; s_knt = 0; s_fail = 0; s_swaps = 0; s_np = 0; }
+    __syncthreads();
+    for (int round = 0; round < 4 * jw + 8; ++round) {
+        if (tid == 0) {
+            while (s_kb >= 0 && s_knt < jw) {
+                const int kb = s_kb;
+                const int b0 = (bsz[kb] == 0) ? kb - 1 : kb;
+                if (st[b0] != ST_U) break;
+                const int bulge = (b0 != kb);
+                bool dfl;
+                if (!bulge) {
+                    s_knt += 1;
+                    double tmp = fabs(s) + fabs(Hw[GQZ_IX(kb, kb, ld)]);
+                    dfl = fabs(s * U[GQZ_IX(0, kb, ld)]) <= fmax(GQZ_ULP * tmp, smlnum);
+                } else {
+                    s_knt += 2;
+                    double h00 = Hw[GQZ_IX(kb - 1, kb - 1, ld)], h10 = Hw[GQZ_IX(kb, kb - 1, ld)];
+                    double h01 = Hw[GQZ_IX(kb - 1, kb, ld)], h11 = Hw[GQZ_IX(kb, kb, ld)];
+                    double rtdet;
+                    if (h00 == 0.0 && h10 == 0.0) rtdet = 0.0;
+                    else if (fabs(h00) >= fabs(h10)) rtdet = sqrt(fabs(h00)) * sqrt(fabs(h11 - (h10 / h00) * h01));
+                    else rtdet = sqrt(fabs(h10)) * sqrt(fabs(h01 - (h00 / h10) * h11));
+                    double tmp = fabs(s) + rtdet;
+                    dfl = fmax(fabs(s * U[GQZ_IX(0, kb, ld)]), fabs(s * U[GQZ_IX(0, kb - 1, ld)])) <=
+                          fmax(GQZ_ULP * tmp, smlnum);
+                }
+                if (dfl) { st[b0] = ST_D; s_kb = b0 - 1; }
+                else {
+                    st[b0] = (b0 == s_ilst) ? ST_A : ST_X;
+                    tgt[b0] = (short)s_ilst;
+                    s_ilst += bulge ? 2 : 1;
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- pair detection, numbered by position: per-lane chunk counts + exclusive scan ----
+        const int kbnow = s_kb;
+        const int npos = kbnow + 1;
+        const int chunk = (npos + nt - 1) / nt;
+        const int p0 = tid * chunk, p1 = min(npos, p0 + chunk);
+        int cnt = 0;
+        for (int p = p0; p < p1; ++p) {
+            if (bsz[p] == 0 || st[p] != ST_X || p <= tgt[p] || p == 0) continue;
+            const int q = (bsz[p - 1] != 0) ? p - 1 : p - 2;
+            if (q >= 0 && st[q] == ST_U) ++cnt;
+        }
+        scan[tid] = cnt;
+        __syncthreads();
+        for (int off = 1; off < nt; off <<= 1) {
+            const int v = (tid >= off) ? scan[tid - off] : 0;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        int k = scan[tid] - cnt;
+        for (int p = p0; p < p1; ++p) {
+            if (bsz[p] == 0 || st[p] != ST_X || p <= tgt[p] || p == 0) continue;
+            const int q = (bsz[p - 1] != 0) ? p - 1 : p - 2;
+            if (q < 0 || st[q] != ST_U) continue;
+            pj1[k] = (short)q; pn1[k] = bsz[q]; pn2[k] = bsz[p]; ++k;
+        }
+        if (tid == nt - 1) s_np = scan[nt - 1];
+        __syncthreads();
+        const int np = s_np;
+        if (np == 0) break;
+        // ---- swap factors of every pair (every workgroup, identically) ----
+        for (int q = tid; q < np; q += nt) {
+            const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
+            double rec[PREC];
+            if (n1 == 1 && n2 == 1) pair_swap<1, 1>(Hw, Tw, ld, j1, rec);
+            else if (n1 == 1 && n2 == 2) pair_swap<1, 2>(Hw, Tw, ld, j1, rec);
+            else if (n1 == 2 && n2 == 1) pair_swap<2, 1>(Hw, Tw, ld, j1, rec);
+            else pair_swap<2, 2>(Hw, Tw, ld, j1, rec);
+            const int m = n1 + n2;
+            pfl[q] = rec[64] != 0.0;
+            if (!pfl[q]) {
+                for (int e = 0; e < m * m; ++e) { pq[q][e] = rec[e]; pq[q][16 + e] = rec[16 + e]; }
+            }
+            if (q % gridDim.x == blockIdx.x) {
+                double *g = pscr + (size_t)PREC * q;
+                g[64] = rec[64];
+                if (!pfl[q]) for (int e = 0; e < m * m; ++e) { g[32 + e] = rec[32 + e]; g[48 + e] = rec[48 + e]; }
+            }
+        }
+        __syncthreads();
+        grid.sync();
+        // ---- rows of every pair (cols right of its block) and U columns ----
+        const int d1 = 3 * jw;
+        const double inv1 = 1.0 / d1;
+        for (int idx = gtid; idx < np * d1; idx += gnt) {
+            const int q = (int)((idx + 0.5) * inv1), r = idx - q * d1;
+            if (pfl[q]) continue;
+            const int j1 = pj1[q], m = pn1[q] + pn2[q];
+            const double *Qa = pq[q];
+            double x[4], y[4];
+            if (r < 2 * jw) {
+                double *M = r < jw ? Hw : Tw;
+                const int col = r < jw ? r : r - jw;
+                if (col < j1 + m) continue;
+                for (int i = 0; i < m; ++i) x[i] = M[GQZ_IX(j1 + i, col, ld)];
+                for (int i = 0; i < m; ++i) { double a = 0; for (int kk = 0; kk < m; ++kk) a += Qa[kk + m * i] * x[kk]; y[i] = a; }
+                for (int i = 0; i < m; ++i) M[GQZ_IX(j1 + i, col, ld)] = y[i];
+            } else {
+                const int row = r - 2 * jw;
+                for (int j = 0; j < m; ++j) x[j] = U[GQZ_IX(row, j1 + j, ld)];
+                for (int j = 0; j < m; ++j) { double a = 0; for (int kk = 0; kk < m; ++kk) a += x[kk] * Qa[kk + m * j]; y[j] = a; }
+                for (int j = 0; j < m; ++j) U[GQZ_IX(row, j1 + j, ld)] = y[j];
+            }
+        }
+        grid.sync();
+        // ---- columns of every pair (rows above its block), V, and the new diagonal blocks ----
+        const int d2 = 3 * jw + 16;
+        const double inv2 = 1.0 / d2;
+        for (int idx = gtid; idx < np * d2; idx += gnt) {
+            const int q = (int)((idx + 0.5) * inv2), r = idx - q * d2;
+            if (pfl[q]) continue;
+            const int j1 = pj1[q], m = pn1[q] + pn2[q];
+            const double *Za = pq[q] + 16;
+            double x[4], y[4];
+            if (r >= 3 * jw) {
+                const int e = r - 3 * jw;
+                if (e < m * m) {
+                    const double *g = pscr + (size_t)PREC * q;
+                    const int i = e % m, j = e / m;
+                    Hw[GQZ_IX(j1 + i, j1 + j, ld)] = g[32 + e];
+                    Tw[GQZ_IX(j1 + i, j1 + j, ld)] = g[48 + e];
+                }
+                continue;
+            }
+            double *M; int row;
+            if (r < jw) { M = Hw; row = r; if (row >= j1) continue; }
+            else if (r < 2 * jw) { M = Tw; row = r - jw; if (row >= j1) continue; }
+            else { M = V; row = r - 2 * jw; }
+            for (int j = 0; j < m; ++j) x[j] = M[GQZ_IX(row, j1 + j, ld)];
+            for (int j = 0; j < m; ++j) { double a = 0; for (int kk = 0; kk < m; ++kk) a += x[kk] * Za[kk + m * j]; y[j] = a; }
+            for (int j = 0; j < m; ++j) M[GQZ_IX(row, j1 + j, ld)] = y[j];
+        }
+        grid.sync();
+        // ---- block-structure update (every workgroup, one lane per pair) ----
+        for (int q = tid; q < np; q += nt) {
+            if (pfl[q]) { s_fail = 1; continue; }
+            const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
+            const int t = tgt[j1 + n1];
+            atomicAdd(&s_swaps, 1);
+            const bool msplit = (n2 == 2) && Hw[GQZ_IX(j1 + 1, j1, ld)] == 0.0;
+            const bool psplit = (n1 == 2) && Hw[GQZ_IX(j1 + n2 + 1, j1 + n2, ld)] == 0.0;
+            if (msplit) {
+                bsz[j1] = 1; bsz[j1 + 1] = 1;
+                st[j1] = (j1 == t) ? ST_A : ST_X; tgt[j1] = (short)t;
+                st[j1 + 1] = (j1 + 1 == t + 1) ? ST_A : ST_X; tgt[j1 + 1] = (short)(t + 1);
+            } else {
+                bsz[j1] = (unsigned char)n2;
+                if (n2 == 2) bsz[j1 + 1] = 0;
+                st[j1] = (j1 == t) ? ST_A : ST_X; tgt[j1] = (short)t;
+            }
+            const int pb = j1 + n2;
+            if (n1 == 2 && !psplit) { bsz[pb] = 2; bsz[pb + 1] = 0; st[pb] = ST_U; }
+            else if (n1 == 2) { bsz[pb] = 1; bsz[pb + 1] = 1; st[pb] = ST_U; st[pb + 1] = ST_U; }
+            else { bsz[pb] = 1; st[pb] = ST_U; }
+        }
+        __syncthreads();
+        if (s_fail) break;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        int kbc = s_kb + 1;
+        if (!s_fail && kbc < s_ilst) kbc = s_ilst;
+        out[0] = kbc; out[1] = s_swaps; out[2] = s_fail; out[3] = (s != 0.0);
+    }
+}
+
+// grid of the cooperative reorder for a window of order jw (0: use the single-workgroup kernel)
+int gqz_aed_reorder_coop_grid(int jw) {
+    if (jw < 160 || jw > COOP_MAXW) return 0;
+    return std::min(128, std::max(16, jw / 6));
 }
 
 // eigenvalues of the window's diagonal blocks (the shifts and the deflated eigenvalues)
-extern "C" __global__ void __launch_bounds__(256) k_aed_eigs(const double *Hw, const double *Tw, int jw,
-                                                             double *war, double *wai, double *wb) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jw; j += gridDim.x * blockDim.x) {
-        const bool lower2 = (j > 0) && Hw[GQZ_IX(j, j - 1, jw)] != 0.0;
-        const bool upper2 = (j + 1 < jw) && Hw[GQZ_IX(j + 1, j, jw)] != 0.0;
+__device__ void aed_eigs_body(const double *Hw, const double *Tw, int ld, int jw, double *war, double *wai,
+                              double *wb, int j0, int stride) {
+    for (int j = j0; j < jw; j += stride) {
+        const bool lower2 = (j > 0) && Hw[GQZ_IX(j, j - 1, ld)] != 0.0;
+        const bool upper2 = (j + 1 < jw) && Hw[GQZ_IX(j + 1, j, ld)] != 0.0;
         if (lower2) continue;
         if (upper2) {
-            double A[4] = { Hw[GQZ_IX(j, j, jw)], Hw[GQZ_IX(j + 1, j, jw)], Hw[GQZ_IX(j, j + 1, jw)], Hw[GQZ_IX(j + 1, j + 1, jw)] };
-            double B[4] = { Tw[GQZ_IX(j, j, jw)], 0.0, Tw[GQZ_IX(j, j + 1, jw)], Tw[GQZ_IX(j + 1, j + 1, jw)] };
+            double A[4] = { Hw[GQZ_IX(j, j, ld)], Hw[GQZ_IX(j + 1, j, ld)], Hw[GQZ_IX(j, j + 1, ld)], Hw[GQZ_IX(j + 1, j + 1, ld)] };
+            double B[4] = { Tw[GQZ_IX(j, j, ld)], 0.0, Tw[GQZ_IX(j, j + 1, ld)], Tw[GQZ_IX(j + 1, j + 1, ld)] };
             double er[2], ei[2];
             d_eig22(A[0], A[1], A[2], A[3], B[0], B[2], B[3], er, ei);
             double b0 = B[0], b1 = B[3];
@@ -212,9 +450,14 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_eigs(const double *Hw, c
                 war[j + 1] = er[1] * b1; wai[j + 1] = 0.0; wb[j + 1] = b1;
             }
         } else {
-            war[j] = Hw[GQZ_IX(j, j, jw)]; wai[j] = 0.0; wb[j] = Tw[GQZ_IX(j, j, jw)];
+            war[j] = Hw[GQZ_IX(j, j, ld)]; wai[j] = 0.0; wb[j] = Tw[GQZ_IX(j, j, ld)];
         }
     }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_aed_eigs(const double *Hw, const double *Tw, int jw,
+                                                             double *war, double *wai, double *wb) {
+    aed_eigs_body(Hw, Tw, jw, jw, war, wai, wb, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Spike reflector and RQ update (qzearly.f:372-436 with PARA_AGGRQUP = .TRUE., kkqzconf.f:138):
@@ -225,18 +468,14 @@ extern "C" __global__ void __launch_bounds__(256) k_aed_eigs(const double *Hw, c
 //   (T becomes Hessenberg, the rank-one term lives in the last column), the second sweep
 //   (i = kbc-2 .. 0) removes the subdiagonal.  Hw's block columns and V see the same rotations.
 // One workgroup; kbc <= AED_MAXW.  Rotations are kept in scr (4 kbc doubles).
-extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, double *Tw, double *U, double *V,
-                                                               int jw, const double *sptr, double *scr,
-                                                               const int *out) {
-    __shared__ double sv[AED_MAXW], sw[AED_MAXW];
-    __shared__ double red[1024];
+// sv, sw: kbc doubles each, red: blockDim.x doubles (LDS); scr: 4 kbc doubles
+__device__ void aed_spike_body(double *Hw, double *Tw, double *U, double *V, int ld, int jw, double s, int kbc,
+                               double *scr, double *sv, double *sw, double *red) {
     __shared__ double s_tau, s_cs, s_sn;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double s = sptr ? *sptr : 0.0;
-    const int kbc = out[0];
     if (kbc <= 0 || s == 0.0 || kbc == jw) return;
     // ---- reflector (DLARFG on s U(0, 0:kbc)) ----
-    for (int i = tid; i < kbc; i += nt) sv[i] = s * U[GQZ_IX(0, i, jw)];
+    for (int i = tid; i < kbc; i += nt) sv[i] = s * U[GQZ_IX(0, i, ld)];
     __syncthreads();
     double ss = 0.0;
     for (int i = 1 + tid; i < kbc; i += nt) ss += sv[i] * sv[i];
@@ -262,7 +501,7 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
     //      w = -tau T(0:kbc,0:kbc)^T v (T upper triangular) ----
     for (int idx = tid; idx < jw + (jw - kbc) + jw + kbc; idx += nt) {
         if (idx < jw + (jw - kbc)) {
-            double *p = (idx < jw) ? Hw + GQZ_IX(0, idx, jw) : Tw + GQZ_IX(0, kbc + idx - jw, jw);
+            double *p = (idx < jw) ? Hw + GQZ_IX(0, idx, ld) : Tw + GQZ_IX(0, kbc + idx - jw, ld);
             double t = 0.0;
             for (int i = 0; i < kbc; ++i) t += sv[i] * p[i];
             t *= tau;
@@ -270,13 +509,13 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
         } else if (idx < 3 * jw - kbc) {
             const int row = idx - (2 * jw - kbc);
             double t = 0.0;
-            for (int j = 0; j < kbc; ++j) t += U[GQZ_IX(row, j, jw)] * sv[j];
+            for (int j = 0; j < kbc; ++j) t += U[GQZ_IX(row, j, ld)] * sv[j];
             t *= tau;
-            for (int j = 0; j < kbc; ++j) U[GQZ_IX(row, j, jw)] -= t * sv[j];
+            for (int j = 0; j < kbc; ++j) U[GQZ_IX(row, j, ld)] -= t * sv[j];
         } else {
             const int j = idx - (3 * jw - kbc);
             double t = 0.0;
-            for (int i = 0; i <= j; ++i) t += Tw[GQZ_IX(i, j, jw)] * sv[i];
+            for (int i = 0; i <= j; ++i) t += Tw[GQZ_IX(i, j, ld)] * sv[i];
             sw[j] = -tau * t;
         }
     }
@@ -297,65 +536,159 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
     for (int row = tid; row < kbc; row += nt) {
         const int i0 = max(0, row - 1);
         if (i0 >= kbc - 1) continue;
-        double x = Tw[GQZ_IX(row, i0, jw)];
+        double x = Tw[GQZ_IX(row, i0, ld)];
         for (int i = i0; i < kbc - 1; ++i) {
-            double y = Tw[GQZ_IX(row, i + 1, jw)];
-            Tw[GQZ_IX(row, i, jw)] = rc[i] * x + rs[i] * y;
+            double y = Tw[GQZ_IX(row, i + 1, ld)];
+            Tw[GQZ_IX(row, i, ld)] = rc[i] * x + rs[i] * y;
             x = rc[i] * y - rs[i] * x;
         }
-        Tw[GQZ_IX(row, kbc - 1, jw)] = x;
+        Tw[GQZ_IX(row, kbc - 1, ld)] = x;
     }
     __syncthreads();
     // rank-one term now lives in the last column: T(:, kbc-1) += w(kbc-1) v
-    for (int i = tid; i < kbc; i += nt) Tw[GQZ_IX(i, kbc - 1, jw)] += sw[kbc - 1] * sv[i];
+    for (int i = tid; i < kbc; i += nt) Tw[GQZ_IX(i, kbc - 1, ld)] += sw[kbc - 1] * sv[i];
     // H block columns (rows 0..kbc-1; H below row kbc is zero in these columns) and V (all rows)
     for (int row = tid; row < kbc + jw; row += nt) {
         double *M = row < kbc ? Hw : V;
         const int r = row < kbc ? row : row - kbc;
-        double x = M[GQZ_IX(r, 0, jw)];
+        double x = M[GQZ_IX(r, 0, ld)];
         for (int i = 0; i < kbc - 1; ++i) {
-            double y = M[GQZ_IX(r, i + 1, jw)];
-            M[GQZ_IX(r, i, jw)] = rc[i] * x + rs[i] * y;
+            double y = M[GQZ_IX(r, i + 1, ld)];
+            M[GQZ_IX(r, i, ld)] = rc[i] * x + rs[i] * y;
             x = rc[i] * y - rs[i] * x;
         }
-        M[GQZ_IX(r, kbc - 1, jw)] = x;
+        M[GQZ_IX(r, kbc - 1, ld)] = x;
     }
     __syncthreads();
     // ---- second sweep: i = kbc-2 .. 0 removes T(i+1, i) (qzearly.f:421-429) ----
     for (int i = kbc - 2; i >= 0; --i) {
         if (tid == 0) {
             double c, sn, r;
-            d_lartg(Tw[GQZ_IX(i + 1, i + 1, jw)], Tw[GQZ_IX(i + 1, i, jw)], c, sn, r);
-            Tw[GQZ_IX(i + 1, i + 1, jw)] = r;
-            Tw[GQZ_IX(i + 1, i, jw)] = 0.0;
+            d_lartg(Tw[GQZ_IX(i + 1, i + 1, ld)], Tw[GQZ_IX(i + 1, i, ld)], c, sn, r);
+            Tw[GQZ_IX(i + 1, i + 1, ld)] = r;
+            Tw[GQZ_IX(i + 1, i, ld)] = 0.0;
             qc[i] = c; qs[i] = -sn;
             s_cs = c; s_sn = -sn;
         }
         __syncthreads();
         const double c = s_cs, sn = s_sn;
         for (int row = tid; row <= i; row += nt) {
-            double x = Tw[GQZ_IX(row, i, jw)], y = Tw[GQZ_IX(row, i + 1, jw)];
-            Tw[GQZ_IX(row, i, jw)] = c * x + sn * y;
-            Tw[GQZ_IX(row, i + 1, jw)] = c * y - sn * x;
+            double x = Tw[GQZ_IX(row, i, ld)], y = Tw[GQZ_IX(row, i + 1, ld)];
+            Tw[GQZ_IX(row, i, ld)] = c * x + sn * y;
+            Tw[GQZ_IX(row, i + 1, ld)] = c * y - sn * x;
         }
         __syncthreads();
     }
     for (int row = tid; row < kbc + jw; row += nt) {
         double *M = row < kbc ? Hw : V;
         const int r = row < kbc ? row : row - kbc;
-        double y = M[GQZ_IX(r, kbc - 1, jw)];
+        double y = M[GQZ_IX(r, kbc - 1, ld)];
         for (int i = kbc - 2; i >= 0; --i) {
-            double x = M[GQZ_IX(r, i, jw)];
-            M[GQZ_IX(r, i + 1, jw)] = qc[i] * y - qs[i] * x;
+            double x = M[GQZ_IX(r, i, ld)];
+            M[GQZ_IX(r, i + 1, ld)] = qc[i] * y - qs[i] * x;
             y = qc[i] * x + qs[i] * y;
         }
-        M[GQZ_IX(r, 0, jw)] = y;
+        M[GQZ_IX(r, 0, ld)] = y;
     }
+    __syncthreads();
+}
+
+extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, double *Tw, double *U, double *V,
+                                                               int jw, const double *sptr, double *scr,
+                                                               const int *out) {
+    __shared__ double sv[AED_MAXW], sw[AED_MAXW];
+    __shared__ double red[1024];
+    aed_spike_body(Hw, Tw, U, V, jw, jw, sptr ? *sptr : 0.0, out[0], scr, sv, sw, red);
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_aed_small: the whole AED of a window of order jw <= AEDS_MAXW in one workgroup, with the
+// window and its U, V resident in LDS (QZEARLY, qzearly.f:1-455): window QZ (double-shift base
+// QZ), deflation test + reordering (the train above), eigenvalues, and - when something deflated
+// or there is no spike (PDHGEQZ3.f:840) - the spike reflector with the RQ update, the HT restore
+// of the undeflatable block and the write-back of the window and of the new spike entry.
+//   out[0] = kbc, out[1] = swaps, out[2] = reorder failed, out[3] = spike present, out[4] = INFO of
+//   the window QZ (non-zero: nothing was written back).
+// U, V (jw x jw, ld jw) receive the window's orthogonal factors for the off-window updates.
+// ---------------------------------------------------------------------------------------------
+#define AEDS_MAXW 64
+#define AEDS_NT 512
+
+size_t gqz_aed_small_lds(int jw) {
+    const int ld = jw + 1;
+    return sizeof(double) * (4 * (size_t)ld * jw + 2 * (size_t)jw + AEDS_NT) + sizeof(WgCtl) + 16;
+}
+int gqz_aed_small_max() { return AEDS_MAXW; }
+
+extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, double *T, int ldh, int kwtop, int jw,
+                                                                  int has_spike, double ttol, double smlnum,
+                                                                  double *U, double *V, double *war, double *wai,
+                                                                  double *wb, double *pscr, double *scr, int *out) {
+    extern __shared__ double lds[];
+    __shared__ double sh[8];
+    __shared__ int res[4];
+    const int ld = jw + 1;
+    double *sH = lds, *sT = sH + (size_t)ld * jw, *sU = sT + (size_t)ld * jw, *sV = sU + (size_t)ld * jw;
+    double *sv = sV + (size_t)ld * jw, *sw = sv + jw, *red = sw + jw;
+    WgCtl *C = (WgCtl *)(red + AEDS_NT);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double s = has_spike ? H[GQZ_IX(kwtop, kwtop - 1, ldh)] : 0.0;
+    for (int idx = tid; idx < jw * jw; idx += nt) {
+        const int i = idx % jw, j = idx / jw;
+        sH[i + ld * j] = (i <= j + 1) ? H[GQZ_IX(kwtop + i, kwtop + j, ldh)] : 0.0;
+        sT[i + ld * j] = (i <= j) ? T[GQZ_IX(kwtop + i, kwtop + j, ldh)] : 0.0;
+        sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
+        sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    WgPencil P;
+    P.H = sH; P.T = sT; P.U = sU; P.V = sV; P.ld = ld; P.ldu = ld; P.n = jw; P.nq = jw;
+    const int info = wg_qz_ds(P, C, 0, jw - 1, war, wai, wb, ttol, smlnum);
+    if (info != 0) {
+        if (tid == 0) { out[0] = jw; out[1] = 0; out[2] = 0; out[3] = (s != 0.0); out[4] = info; }
+        return;
+    }
+    aed_reorder_body<AEDS_MAXW>(sH, sT, sU, sV, ld, jw, s, smlnum, pscr, res);
+    aed_eigs_body(sH, sT, ld, jw, war, wai, wb, tid, nt);
+    const int kbc = res[0];
+    if (kbc < jw || s == 0.0) {
+        if (kbc > 0 && s != 0.0) aed_spike_body(sH, sT, sU, sV, ld, jw, s, kbc, scr, sv, sw, red);
+        if (kbc > 2 && s != 0.0) wg_gghrd(sH, sT, ld, jw, 0, kbc - 1, sU, sV, ld, jw, sh);
+        __syncthreads();
+        for (int idx = tid; idx < jw * jw; idx += nt) {
+            const int i = idx % jw, j = idx / jw;
+            H[GQZ_IX(kwtop + i, kwtop + j, ldh)] = sH[i + ld * j];
+            T[GQZ_IX(kwtop + i, kwtop + j, ldh)] = sT[i + ld * j];
+            U[i + (size_t)jw * j] = sU[i + ld * j];
+            V[i + (size_t)jw * j] = sV[i + ld * j];
+        }
+        if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = s * sU[0];
+    }
+    if (tid == 0) { out[0] = res[0]; out[1] = res[1]; out[2] = res[2]; out[3] = res[3]; out[4] = 0; }
+}
+
+void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwtop, int jw, int has_spike,
+                          double ttol, double smlnum, double *U, double *V, double *war, double *wai, double *wb,
+                          double *pscr, double *scr, int *out) {
+    const size_t lds = gqz_aed_small_lds(jw);
+    static size_t lds_set = 0;
+    if (lds > lds_set) {
+        hipFuncSetAttribute((const void *)k_aed_small, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(k_aed_small, dim3(1), dim3(AEDS_NT), lds, st, H, T, ldh, kwtop, jw, has_spike, ttol, smlnum,
+                       U, V, war, wai, wb, pscr, scr, out);
 }
 
 void gqz_launch_aed_reorder(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
                             const double *sptr, double smlnum, double *pscr, int *out) {
-    hipLaunchKernelGGL(k_aed_reorder, dim3(1), dim3(256), 0, st, Hw, Tw, U, V, jw, sptr, smlnum, pscr, out);
+    const int g = gqz_aed_reorder_coop_grid(jw);
+    if (g > 0) {
+        void *args[] = { &Hw, &Tw, &U, &V, &jw, (void *)&sptr, &smlnum, &pscr, &out };
+        GQZ_HIP(hipLaunchCooperativeKernel((const void *)k_aed_reorder_coop, dim3(g), dim3(COOP_NT), args, 0, st));
+        return;
+    }
+    hipLaunchKernelGGL(k_aed_reorder, dim3(1), dim3(512), 0, st, Hw, Tw, U, V, jw, sptr, smlnum, pscr, out);
 }
 void gqz_launch_aed_eigs(hipStream_t st, const double *Hw, const double *Tw, int jw, double *war, double *wai,
                          double *wb) {

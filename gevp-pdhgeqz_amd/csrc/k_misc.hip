@@ -16,30 +16,42 @@ extern "C" __global__ void __launch_bounds__(1024) k_inf_defl(double *H, double 
                                                               double smlnum, double *ar, double *ai,
                                                               double *be, int *out) {
     __shared__ int s_i[8];
+    __shared__ int s_max;
     __shared__ double s_d[4];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     WgPencil P;
     P.H = H; P.T = T; P.U = Q; P.V = Z; P.ld = ld; P.ldu = ldq; P.n = n; P.nq = nq;
+    // lowest negligible subdiagonal entry (the bottom-up search of PDLASMSUB finds the largest k),
+    // found by all lanes at once
+    if (tid == 0) s_max = ilo;
+    __syncthreads();
+    for (int k = ilo + 1 + tid; k <= kbot; k += nt) {
+        double tst = fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]);
+        if (fabs(H[GQZ_IX(k, k - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) atomicMax(&s_max, k);
+    }
+    __syncthreads();
     if (tid == 0) {
-        int ktop = ilo;
-        for (int k = kbot; k > ilo; --k) {
-            double tst = fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]);
-            if (fabs(H[GQZ_IX(k, k - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) {
-                H[GQZ_IX(k, k - 1, ld)] = 0.0; ktop = k; break;
-            }
-        }
+        const int ktop = s_max;
+        if (ktop > ilo) H[GQZ_IX(ktop, ktop - 1, ld)] = 0.0;
         s_i[0] = ktop; s_i[1] = ktop; s_i[2] = kbot; s_i[3] = 0;
     }
     __syncthreads();
     const double tol = fmax(ttol, smlnum);
     for (;;) {
+        // negligible diagonal entries of T in [kt, kb] are set to zero; s_max = the lowest one
+        if (tid == 0) s_max = -1;
+        __syncthreads();
+        {
+            const int kt = s_i[1], kb = s_i[2];
+            for (int i = kt + tid; i <= kb; i += nt)
+                if (fabs(T[GQZ_IX(i, i, ld)]) <= tol) { T[GQZ_IX(i, i, ld)] = 0.0; atomicMax(&s_max, i); }
+        }
+        __syncthreads();
         if (tid == 0) {
             int kt = s_i[1], kb = s_i[2];
             s_i[4] = 0;   // op: 0 done, 1 colrot bottom, 2 rowrot top, 3 chase
             if (kb >= kt) {
-                int found = -1;
-                for (int i = kb; i >= kt; --i)
-                    if (fabs(T[GQZ_IX(i, i, ld)]) <= tol) { T[GQZ_IX(i, i, ld)] = 0.0; if (found < 0) found = i; }
+                const int found = s_max;
                 if (found >= 0) {
                     double c, s, r;
                     if (kb == kt) {

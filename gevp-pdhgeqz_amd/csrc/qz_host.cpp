@@ -261,12 +261,48 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     size_t jj = (size_t)jw * jw;
     double *Hw = L.ws.alloc(jj), *Tw = L.ws.alloc(jj), *U = L.ws.alloc(jj), *V = L.ws.alloc(jj);
     double *weig = L.ws.alloc(3 * (size_t)jw), *scr = L.ws.alloc(4 * (size_t)jw);
+    int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
+    if (jw <= std::min(nmin, gqz_aed_small_max())) {
+        // the whole AED in one LDS-resident kernel (window QZ, reorder, spike, restore, write-back)
+        double *pscr = L.ws.alloc((size_t)(jw / 2 + 1) * 65);
+        int kk = kt_begin(GQZ_KC_SMALL);
+        gqz_launch_aed_small(L.st, v.H, v.T, v.ld, kwtop, jw, kwtop > ktop ? 1 : 0, R.ttol,
+                             kSafmin * ((double)jw / kUlp), U, V, weig, weig + jw, weig + 2 * jw, pscr, scr,
+                             L.d_out + 16);
+        kt_end(kk, 0.0, 64.0 * jw * jw);
+        HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 5 * sizeof(int), hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipStreamSynchronize(L.st));
+        if (!KT.pend.empty()) kt_drain();
+        R.st.aed_calls++;
+        R.st.small_calls++;
+        if (L.h_out[20] != 0) {     // window QZ failed: soft recovery, no deflation (PDHGEQZ0.f:944-949)
+            L.ws.release(mk);
+            ls = 0;
+            if (level == 0) R.st.t_aed += now() - t0;
+            return 0;
+        }
+        int kbc = L.h_out[16];
+        R.st.swaps += L.h_out[17];
+        R.st.swap_fail += L.h_out[18] ? 1 : 0;
+        const int spike = L.h_out[19];
+        if (kbc < jw || !spike) apply_window(R, v, kwtop, jw, U, V);
+        sh.assign(3 * (size_t)kbc, 0.0);
+        for (int i = 0; i < kbc; ++i) {
+            sh[i] = L.h_buf[i];
+            sh[kbc + i] = L.h_buf[jw + i];
+            sh[2 * kbc + i] = L.h_buf[2 * jw + i];
+        }
+        ls = kbc;
+        L.ws.release(mk);
+        if (level == 0) R.st.t_aed += now() - t0;
+        return jw - kbc;
+    }
     gqz_launch_window_in(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, V);
     View W;
     W.H = Hw; W.T = Tw; W.Q = U; W.Z = V; W.ld = jw; W.ldq = jw; W.n = jw; W.nq = jw;
     W.ar = weig; W.ai = weig + jw; W.be = weig + 2 * jw;
     W.smlnum = kSafmin * ((double)jw / kUlp);
-    int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
     int info;
     if (jw <= nmin) {
         GqzSmallJob J = {};

@@ -195,6 +195,7 @@ static void g2_cols(double *M, int ld, int j, int r0, int r1, const double G[4])
  * (column-major).  Real eigenvalues: A and B upper triangular.  Complex pair: B diagonal
  * with positive entries (the right factor of B's SVD, obtained by one Jacobi rotation of
  * B^T B, and the left factor from the normalised columns of B ZR). */
+#define orc_lartg_v(f, g, c, s, r) orc_lartg((f), (g), &(c), &(s), &(r))
 void orc_lagv2(double *A, int lda, double *B, int ldb, double alphar[2], double alphai[2],
                double beta[2], double QL[4], double ZR[4]) {
     double ar[2], ai[2], bb[2];
@@ -239,31 +240,33 @@ void orc_lagv2(double *A, int lda, double *B, int ldb, double alphar[2], double 
     if (q != 0.0) {
         double theta = 0.5 * (w - p) / q;
         double t = copysign(1.0, theta) / (fabs(theta) + sqrt(1.0 + theta * theta));
-        c2 = 1.0 / sqrt(1.0 + t * t); s2 = t * c2;
+        { double rr; orc_lartg(1.0, t, &c2, &s2, &rr); }
     }
     ZR[0] = c2; ZR[1] = -s2; ZR[2] = s2; ZR[3] = c2;
     g2_cols(A, lda, 0, 0, 2, ZR);
     g2_cols(B, ldb, 0, 0, 2, ZR);
     double b0x = B[IX(0, 0, ldb)], b0y = B[IX(1, 0, ldb)], b1x = B[IX(0, 1, ldb)], b1y = B[IX(1, 1, ldb)];
     double s0 = hypot(b0x, b0y), s1 = hypot(b1x, b1y);
-    double u0x, u0y, u1x, u1y;
+    /* B ZR has orthogonal columns b0, b1.  QL^T is the plane rotation (or, when needed to keep both
+     * diagonal entries positive, reflection) that maps the longer of them onto a coordinate axis.
+     * (Normalising both columns separately, Gram-Schmidt style, leaves an orthogonality defect of
+     * one sign that accumulates linearly over many standardisations.) */
+    double c, s;
+    double m00, m01, m10, m11;
     if (s0 >= s1) {
-        u0x = b0x / s0; u0y = b0y / s0;
-        if (s1 > 0.0) { u1x = b1x / s1; u1y = b1y / s1; } else { u1x = -u0y; u1y = u0x; }
-        /* re-orthogonalise the second column against the first */
-        double d = u1x * u0x + u1y * u0y;
-        u1x -= d * u0x; u1y -= d * u0y;
-        double nn = hypot(u1x, u1y);
-        if (nn > 0.0) { u1x /= nn; u1y /= nn; } else { u1x = -u0y; u1y = u0x; }
+        orc_lartg_v(b0x, b0y, c, s, r);
+        if (r < 0.0) { c = -c; s = -s; }
+        m00 = c; m01 = s;
+        if (-s * b1x + c * b1y >= 0.0) { m10 = -s; m11 = c; }   /* rotation */
+        else { m10 = s; m11 = -c; }                             /* reflection */
     } else {
-        u1x = b1x / s1; u1y = b1y / s1;
-        if (s0 > 0.0) { u0x = b0x / s0; u0y = b0y / s0; } else { u0x = u1y; u0y = -u1x; }
-        double d = u0x * u1x + u0y * u1y;
-        u0x -= d * u1x; u0y -= d * u1y;
-        double nn = hypot(u0x, u0y);
-        if (nn > 0.0) { u0x /= nn; u0y /= nn; } else { u0x = u1y; u0y = -u1x; }
+        orc_lartg_v(b1y, -b1x, c, s, r);
+        if (r < 0.0) { c = -c; s = -s; }
+        m10 = -s; m11 = c;
+        if (c * b0x + s * b0y >= 0.0) { m00 = c; m01 = s; }
+        else { m00 = -c; m01 = -s; }
     }
-    QL[0] = u0x; QL[1] = u0y; QL[2] = u1x; QL[3] = u1y;
+    QL[0] = m00; QL[1] = m01; QL[2] = m10; QL[3] = m11;
     g2_rows(A, lda, 0, 0, 2, QL);
     g2_rows(B, ldb, 0, 0, 2, QL);
     B[IX(1, 0, ldb)] = 0.0; B[IX(0, 1, ldb)] = 0.0;

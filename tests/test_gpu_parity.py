@@ -410,3 +410,24 @@ def test_batched_graph_replay_and_tuner():
         for a, b in zip(outs[0], outs[rep]):
             assert np.array_equal(a, b)
     assert (outs[0][7] == 0).all()
+
+
+@pytest.mark.parametrize("nw", [200, 520])
+def test_hgeqz_large_aed_window(nw):
+    """Large AED windows take the cooperative multi-workgroup reorder and the global-memory
+    spike/restore kernels (windows <= 64 run fully in LDS)."""
+    torch, gqz, O = _mods()
+    n = 1100
+    _note("hgeqz large aed window", n, nw)
+    H, T = QC.pencil_ht_random(n, 21)
+    Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+    Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+    prm = gqz.GqzParams()
+    prm.nw = nw
+    ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", Hd, Td, Qd, Zd, params=prm)
+    assert info == 0 and st["aed_calls"] > 0
+    S, P, Qn, Zn = _host(Hd), _host(Td), _host(Qd), _host(Zd)
+    ra, rb, rq, rz = QC.compres(H, T, S, P, Qn, Zn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+    below, tb, ok2 = QC.schur_structure(S, P)
+    assert below == 0 and tb == 0 and ok2
