@@ -50,10 +50,31 @@ __device__ inline void d_larfg(int n, double &alpha, double *x, int incx, double
     alpha = beta;
 }
 
+// sqrt(a^2 + b^2): the plain formula where neither square can overflow or underflow, hypot elsewhere
+__device__ __forceinline__ double d_hyp2(double a, double b) {
+    const double m = fmax(fabs(a), fabs(b));
+    if (m > 1e-140 && m < 1e140) return sqrt(a * a + b * b);
+    return hypot(a, b);
+}
+
+// DLARFG for a 3-vector [alpha; x1; x2] (same reflector as d_larfg(3, ...), fewer divisions)
 __device__ __forceinline__ void d_larfg3(double &alpha, double &x1, double &x2, double &tau) {
-    double x[2] = { x1, x2 };
-    d_larfg(3, alpha, x, 1, tau);
-    x1 = x[0]; x2 = x[1];
+    const double xn = d_hyp2(x1, x2);
+    if (xn == 0.0) { tau = 0.0; return; }
+    const double beta = -copysign(d_hyp2(alpha, xn), alpha);
+    tau = (beta - alpha) / beta;
+    const double sc = 1.0 / (alpha - beta);
+    x1 *= sc; x2 *= sc;
+    alpha = beta;
+}
+
+// DLARFG for a 2-vector [alpha; x1]
+__device__ __forceinline__ void d_larfg2(double &alpha, double &x1, double &tau) {
+    if (x1 == 0.0) { tau = 0.0; return; }
+    const double beta = -copysign(d_hyp2(alpha, x1), alpha);
+    tau = (beta - alpha) / beta;
+    x1 *= 1.0 / (alpha - beta);
+    alpha = beta;
 }
 
 // Eigenvalues of a 2x2 pencil (A, B), B upper triangular with non-zero diagonal:
@@ -112,7 +133,7 @@ __device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &
     double a = t[8];
     double x[2] = { t[2], t[5] };
     double tau1;
-    d_larfg(3, a, x, 1, tau1);
+    d_larfg3(a, x[0], x[1], tau1);
     double u1[3] = { x[0], x[1], 1.0 };
     for (int i = 0; i < 3; ++i) {
         double s = tau1 * (t[i] * u1[0] + t[i + 3] * u1[1] + t[i + 6] * u1[2]);
@@ -121,7 +142,7 @@ __device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &
     double a2 = t[4];
     double x2[1] = { t[1] };
     double tau2;
-    d_larfg(2, a2, x2, 1, tau2);
+    d_larfg2(a2, x2[0], tau2);
     double w0 = 1.0, w1 = 0.0, w2 = 0.0;
     double s2 = tau2 * (w0 * x2[0] + w1);
     w0 -= s2 * x2[0]; w1 -= s2;

@@ -51,18 +51,85 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
                 }
                 H[GQZ_IX(k + 1, k - 1, ld)] = 0.0; H[GQZ_IX(k + 2, k - 1, ld)] = 0.0;
             }
-            C->v[0] = 1.0; C->v[1] = v1; C->v[2] = v2; C->tau = tau;
-        }
-        __syncthreads();
-        wg_left3(P, k, k, k, P.n, C->v, C->tau);
-        __syncthreads();
-        if (tid == 0) {
+            // the right (opposite) reflector is computed from the 3x3 block of T after the left
+            // reflector, which lane 0 forms privately; lane 0 also updates the 3x3 corners of H and
+            // T (left then right), so that all other entries can be updated in a single phase
+            double t3[9], h3[9];
+            for (int jj = 0; jj < 3; ++jj)
+                for (int ii = 0; ii < 3; ++ii) {
+                    t3[ii + 3 * jj] = T[GQZ_IX(k + ii, k + jj, ld)];
+                    h3[ii + 3 * jj] = H[GQZ_IX(k + ii, k + jj, ld)];
+                }
+            if (tau != 0.0) {
+                for (int jj = 0; jj < 3; ++jj) {
+                    double *tc = t3 + 3 * jj, *hc = h3 + 3 * jj;
+                    double a = tau * (tc[0] + v1 * tc[1] + v2 * tc[2]);
+                    tc[0] -= a; tc[1] -= a * v1; tc[2] -= a * v2;
+                    double b = tau * (hc[0] + v1 * hc[1] + v2 * hc[2]);
+                    hc[0] -= b; hc[1] -= b * v1; hc[2] -= b * v2;
+                }
+            }
             double w[3], tau2;
-            d_invhse3(P.T + GQZ_IX(k, k, P.ld), P.ld, w, tau2);
+            d_invhse3(t3, 3, w, tau2);
+            if (tau2 != 0.0) {
+                for (int ii = 0; ii < 3; ++ii) {
+                    double a = tau2 * (t3[ii] + w[1] * t3[ii + 3] + w[2] * t3[ii + 6]);
+                    t3[ii] -= a; t3[ii + 3] -= a * w[1]; t3[ii + 6] -= a * w[2];
+                    double b = tau2 * (h3[ii] + w[1] * h3[ii + 3] + w[2] * h3[ii + 6]);
+                    h3[ii] -= b; h3[ii + 3] -= b * w[1]; h3[ii + 6] -= b * w[2];
+                }
+            }
+            for (int jj = 0; jj < 3; ++jj)
+                for (int ii = 0; ii < 3; ++ii) {
+                    T[GQZ_IX(k + ii, k + jj, ld)] = t3[ii + 3 * jj];
+                    H[GQZ_IX(k + ii, k + jj, ld)] = h3[ii + 3 * jj];
+                }
+            C->v[0] = 1.0; C->v[1] = v1; C->v[2] = v2; C->tau = tau;
             C->w[0] = 1.0; C->w[1] = w[1]; C->w[2] = w[2]; C->tau2 = tau2;
         }
         __syncthreads();
-        wg_right3(P, k, 0, min(k + 4, l + 1), k + 3, C->w, C->tau2);
+        {
+            // one phase: left reflector on rows k..k+2 (columns right of the corner, and U), right
+            // reflector on columns k..k+2 (rows above the corner, H's fill row k+3, and V); the
+            // two sets are disjoint outside the corner
+            const double tau = C->tau, v1 = C->v[1], v2 = C->v[2];
+            const double tau2 = C->tau2, w1 = C->w[1], w2 = C->w[2];
+            const int ld = P.ld, ldu = P.ldu;
+            const int nlh = P.n - (k + 3), nlt = nlh, nu = P.U ? P.nq : 0;
+            const int hbot = min(k + 4, l + 1);           // rows [0, hbot) of H see the right reflector
+            const int nrh = k + (hbot > k + 3 ? 1 : 0), nrt = k, nv = P.V ? P.nq : 0;
+            const int tot = nlh + nlt + nu + nrh + nrt + nv;
+            for (int idx = tid; idx < tot; idx += blockDim.x) {
+                int t = idx;
+                if (t < nlh + nlt) {
+                    if (tau == 0.0) continue;
+                    double *M = t < nlh ? P.H : P.T;
+                    const int j = k + 3 + (t < nlh ? t : t - nlh);
+                    double *p = M + GQZ_IX(k, j, ld);
+                    double a = tau * (p[0] + v1 * p[1] + v2 * p[2]);
+                    p[0] -= a; p[1] -= a * v1; p[2] -= a * v2;
+                    continue;
+                }
+                t -= nlh + nlt;
+                if (t < nu) {
+                    if (tau == 0.0) continue;
+                    double *x = P.U + GQZ_IX(t, k, ldu);
+                    double a0 = x[0], b0 = x[ldu], c0 = x[2 * (size_t)ldu];
+                    double a = tau * (a0 + v1 * b0 + v2 * c0);
+                    x[0] = a0 - a; x[ldu] = b0 - a * v1; x[2 * (size_t)ldu] = c0 - a * v2;
+                    continue;
+                }
+                t -= nu;
+                if (tau2 == 0.0) continue;
+                double *x; int ldx;
+                if (t < nrh) { x = P.H + GQZ_IX(t < k ? t : k + 3, k, ld); ldx = ld; }
+                else if (t < nrh + nrt) { x = P.T + GQZ_IX(t - nrh, k, ld); ldx = ld; }
+                else { x = P.V + GQZ_IX(t - nrh - nrt, k, ldu); ldx = ldu; }
+                double a0 = x[0], b0 = x[ldx], c0 = x[2 * (size_t)ldx];
+                double a = tau2 * (a0 + w1 * b0 + w2 * c0);
+                x[0] = a0 - a; x[ldx] = b0 - a * w1; x[2 * (size_t)ldx] = c0 - a * w2;
+            }
+        }
         __syncthreads();
     }
     // push-off at the bottom
@@ -111,6 +178,28 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
 }
 
 // lane-0 decision step of the base QZ; fills C->op and its parameters.
+// parallel part of the deflation decision: the largest j in (lo, ilast] with a negligible H(j, j-1)
+// (PDLASMSUB.f:368; lo when there is none) and, in [that j, ilast], the largest i with a
+// negligible T(i, i) (-1 when none); negligible T(i, i) are set to zero
+static __device__ void ds_scan(const WgPencil &P, WgCtl *C, double ttol, double smlnum) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double *H = P.H;
+    double *T = P.T;
+    const int ld = P.ld;
+    const int lo = C->lo, il = C->ilast;
+    if (tid == 0) { C->scan_f = lo; C->scan_t = -1; }
+    __syncthreads();
+    for (int j = lo + 1 + tid; j <= il; j += nt) {
+        double tst = fabs(H[GQZ_IX(j - 1, j - 1, ld)]) + fabs(H[GQZ_IX(j, j, ld)]);
+        if (fabs(H[GQZ_IX(j, j - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) atomicMax(&C->scan_f, j);
+    }
+    __syncthreads();
+    const double tol = fmax(ttol, smlnum);
+    for (int i = C->scan_f + tid; i <= il; i += nt)
+        if (fabs(T[GQZ_IX(i, i, ld)]) <= tol) { T[GQZ_IX(i, i, ld)] = 0.0; atomicMax(&C->scan_t, i); }
+    __syncthreads();
+}
+
 static __device__ void ds_decide(const WgPencil &P, WgCtl *C, double *ar, double *ai, double *be,
                           double ttol, double smlnum) {
     double *H = P.H, *T = P.T;
@@ -119,19 +208,12 @@ static __device__ void ds_decide(const WgPencil &P, WgCtl *C, double *ar, double
     if (C->ilast < C->lo) { C->op = OP_SEGDONE; return; }
     if (++C->its > C->maxit) { C->op = OP_FAIL; C->info = C->ilast + 1; return; }
     int lo = C->lo, il = C->ilast;
-    int f = lo;
-    for (int j = il; j > lo; --j) {
-        double tst = fabs(H[GQZ_IX(j - 1, j - 1, ld)]) + fabs(H[GQZ_IX(j, j, ld)]);
-        if (fabs(H[GQZ_IX(j, j - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) {
-            H[GQZ_IX(j, j - 1, ld)] = 0.0; f = j; break;
-        }
-    }
+    // the bottom-up search for a negligible subdiagonal entry and the scan of diag(T)
+    // (PDHGEQZ7.f:380) were done by the whole workgroup (ds_scan)
+    const int f = C->scan_f;
+    if (f > lo) H[GQZ_IX(f, f - 1, ld)] = 0.0;
     C->ifirst = f;
-    // zeros on diag(T) (PDHGEQZ7.f:380)
-    double tol = fmax(ttol, smlnum);
-    int found = -1;
-    for (int i = il; i >= f; --i)
-        if (fabs(T[GQZ_IX(i, i, ld)]) <= tol) { T[GQZ_IX(i, i, ld)] = 0.0; if (found < 0) found = i; }
+    const int found = C->scan_t;
     if (found >= 0) {
         C->iiter = 0;
         if (il == f) {
@@ -243,6 +325,7 @@ static __device__ int wg_qz_ds(const WgPencil &P, WgCtl *C, int ilo, int ihi, do
         __syncthreads();
         if (C->op == OP_SEGDONE) break;
         for (;;) {
+            ds_scan(P, C, ttol, smlnum);
             if (tid == 0) ds_decide(P, C, ar, ai, be, ttol, smlnum);
             __syncthreads();
             int op = C->op;

@@ -49,3 +49,11 @@ void gqz_set_error(const char *msg);    // text returned by gqz_last_error
         if (e_ != hipSuccess)                                                                   \
             throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));          \
     } while (0)
+
+// learned parameter policy (k_mlp.hip): FP32 perceptron 12 -> 64 -> 64 -> 3
+int gqz_mlp_nparams();
+void gqz_launch_mlp_fwd(hipStream_t st, const float *P, const float *X, int B, float *Y);
+void gqz_launch_mlp_grad(hipStream_t st, const float *P, const float *X, const float *Tg, int B, float *G,
+                         float *loss);
+void gqz_launch_mlp_adam(hipStream_t st, float *P, float *G, float *M, float *V, int step, float lr);
+void gqz_launch_fill_f(hipStream_t st, float *x, int n, float v);

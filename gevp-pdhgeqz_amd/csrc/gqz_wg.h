@@ -23,6 +23,7 @@ struct WgCtl {
     int op, i0, i1, i2, i3, i4;
     int lo, ilast, ifirst, iiter, its, maxit, info;
     int sp;
+    int scan_f, scan_t;      // results of the parallel deflation scans of wg_qz_ds
     int stk_lo[96], stk_hi[96];
     double sr[2], si[2], sb[2];
     double red[64];

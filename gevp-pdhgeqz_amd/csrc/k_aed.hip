@@ -612,7 +612,7 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
 // U, V (jw x jw, ld jw) receive the window's orthogonal factors for the off-window updates.
 // ---------------------------------------------------------------------------------------------
 #define AEDS_MAXW 64
-#define AEDS_NT 512
+#define AEDS_NT 256
 
 size_t gqz_aed_small_lds(int jw) {
     const int ld = jw + 1;

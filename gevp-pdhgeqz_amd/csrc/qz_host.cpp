@@ -64,6 +64,10 @@ struct Lib {
     double *d_norm = nullptr;
     gqz_policy_fn pol = nullptr;
     void *pol_user = nullptr;
+    // learned policy: parameters, one feature row and one output row on the device
+    float *d_mlp = nullptr, *d_mlp_x = nullptr, *d_mlp_y = nullptr;
+    float *h_mlp_io = nullptr;       // pinned: 12 features + 3 outputs
+    bool mlp_on = false;
     std::vector<gqz_trace_rec> trace;
     int trace_cap = 0;
     std::string err;
@@ -514,6 +518,16 @@ void sweep(Run &R, const View &v, int ktop, int kbot, std::vector<double> &sh, i
     if (level == 0) R.st.t_sweep += now() - t0;
 }
 
+// one inference of the learned policy on the library stream
+void mlp_infer(const float feat[12], float out[3]) {
+    std::memcpy(L.h_mlp_io, feat, 12 * sizeof(float));
+    HIPCHK(hipMemcpyAsync(L.d_mlp_x, L.h_mlp_io, 12 * sizeof(float), hipMemcpyHostToDevice, L.st));
+    gqz_launch_mlp_fwd(L.st, L.d_mlp, L.d_mlp_x, 1, L.d_mlp_y);
+    HIPCHK(hipMemcpyAsync(L.h_mlp_io + 12, L.d_mlp_y, 3 * sizeof(float), hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipStreamSynchronize(L.st));
+    std::memcpy(out, L.h_mlp_io + 12, 3 * sizeof(float));
+}
+
 // deflation-history features for the parameter policy (12 values, all O(1))
 void features(Run &R, int nh, float f[12]) {
     int n = std::max(1, R.n_top);
@@ -586,8 +600,21 @@ int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
         float feat[12];
         if (top) {
             features(R, nh, feat);
-            if (R.prm.use_policy && L.pol) {
-                int a = -1, b = -1, c = -1;
+            if (R.prm.use_policy && L.mlp_on) {
+                // learned policy: log2 multipliers of the PIPARMQ values from the MLP kernel
+                float o[3];
+                mlp_infer(feat, o);
+                auto mult = [](float x, float lo, float hi) {
+                    return std::exp2((double)std::min(hi, std::max(lo, std::isfinite(x) ? x : 0.0f)));
+                };
+                int a = (int)std::lround(ns_def * mult(o[0], -2.0f, 1.0f));
+                int b = (int)std::lround(nw_def * mult(o[1], -2.0f, 1.0f));
+                int c = (int)std::lround(nib_def * mult(o[2], -2.0f, 2.0f));
+                ns_it = std::max(2, a - a % 2);
+                nw_it = std::max(8, b);
+                nib_it = std::min(100, std::max(1, c));
+            } else if (R.prm.use_policy && L.pol) {
+                int a = ns_def, b = nw_def, c = nib_def;
                 L.pol(L.pol_user, feat, nh, &a, &b, &c);
                 if (a > 0) ns_it = std::max(2, a - a % 2);
                 if (b > 0) nw_it = std::max(4, b);
@@ -698,6 +725,9 @@ int gqz_finalize(void) {
     if (L.ws.base) hipFree(L.ws.base);
     L.ws = Arena();
     hipFree(L.d_out); hipFree(L.d_norm); hipHostFree(L.h_out); hipHostFree(L.h_buf);
+    if (L.d_mlp) { hipFree(L.d_mlp); hipFree(L.d_mlp_x); hipFree(L.d_mlp_y); hipHostFree(L.h_mlp_io); }
+    L.d_mlp = L.d_mlp_x = L.d_mlp_y = L.h_mlp_io = nullptr;
+    L.mlp_on = false;
     hipStreamDestroy(L.st);
     L.st = nullptr; L.dev = -1;
     return 0;
@@ -733,6 +763,69 @@ int gqz_ktime_get(gqz_ktime *out, int cap) {
         out[c].flops = KT.flops[c]; out[c].bytes = KT.bytes[c];
     }
     return k;
+}
+
+int gqz_mlp_nparams_c(void) { return gqz_mlp_nparams(); }
+
+int gqz_mlp_forward_dev(const float *params, const float *x, int batch, float *y) {
+    if (batch < 0 || !params || (batch > 0 && (!x || !y))) return -1;
+    try {
+        if (check_init()) return -100;
+        gqz_launch_mlp_fwd(L.st, params, x, batch, y);
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_mlp_grad_dev(const float *params, const float *x, const float *target, int batch, float *grad,
+                     float *loss) {
+    if (batch <= 0 || !params || !x || !target || !grad) return -1;
+    try {
+        if (check_init()) return -100;
+        gqz_launch_mlp_grad(L.st, params, x, target, batch, grad, loss);
+        HIPCHK(hipStreamSynchronize(L.st));
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_mlp_train_step_dev(float *params, float *grad, float *m, float *v, int step, float lr, const float *x,
+                           const float *target, int batch, float *loss) {
+    if (batch <= 0 || step < 1 || !params || !grad || !m || !v || !x || !target) return -1;
+    try {
+        if (check_init()) return -100;
+        gqz_launch_mlp_grad(L.st, params, x, target, batch, grad, loss);
+        gqz_launch_mlp_adam(L.st, params, grad, m, v, step, lr);
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+int gqz_mlp_policy(const float *params_host, int enable) {
+    try {
+        if (check_init()) return -100;
+        if (!L.d_mlp) {
+            HIPCHK(hipMalloc((void **)&L.d_mlp, sizeof(float) * (gqz_mlp_nparams() + 16)));
+            HIPCHK(hipMalloc((void **)&L.d_mlp_x, sizeof(float) * 16));
+            HIPCHK(hipMalloc((void **)&L.d_mlp_y, sizeof(float) * 16));
+            HIPCHK(hipHostMalloc((void **)&L.h_mlp_io, sizeof(float) * 16, hipHostMallocDefault));
+        }
+        if (params_host) {
+            HIPCHK(hipMemcpy(L.d_mlp, params_host, sizeof(float) * gqz_mlp_nparams(), hipMemcpyHostToDevice));
+        }
+        L.mlp_on = enable && (params_host || L.mlp_on);
+        return 0;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
 }
 
 int gqz_set_policy(gqz_policy_fn fn, void *user) { L.pol = fn; L.pol_user = user; return 0; }

@@ -40,8 +40,9 @@ typedef struct {
     double t_iter;
 } gqz_trace_rec;
 
-/* Parameter policy: called at each top-level iteration with the 12 deflation-history features;
- * writes ns, nw, nibble (<= 0 keeps the default). */
+/* Parameter policy: called at each top-level iteration with the 12 deflation-history features and
+ * the order nh of the active block; ns, nw, nibble hold the default values (PIPARMQ of the block
+ * passed to the QZ call) on entry and may be overwritten (<= 0 keeps the default). */
 typedef void (*gqz_policy_fn)(void *user, const float feat[12], int nh, int *ns, int *nw, int *nibble);
 
 /* ---- library ---------------------------------------------------------------------------- */
@@ -127,6 +128,26 @@ int gqz_chase_window_dev(double *H, double *T, int ld, int p, int w, int nb, int
  * block's eigenvalues; returns the kernel's INFO. */
 int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,
                      double *alphar, double *alphai, double *beta, double ttol);
+
+/* ---- learned parameter policy (replaces PILAENVX ISPEC 51/52/53, pilaenvx.f:1; piparmq.f:440-517) ----
+ * A 12 -> 64 -> 64 -> 3 perceptron, GELU, FP32, parameters as one block of gqz_mlp_nparams_c()
+ * floats: W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] (row-major).  Inputs: the 12
+ * deflation-history features of a top-level iteration (gqz_trace_rec.feat); outputs: log2
+ * multipliers of the default NS, NW and NIBBLE (clamped to [-2, 1], [-2, 1], [-2, 2]).
+ * All pointers of the _dev calls are device pointers. */
+int gqz_mlp_nparams_c(void);
+int gqz_mlp_forward_dev(const float *params, const float *x, int batch, float *y);
+/* adds dL/dparams of L = mean over the batch and the 3 outputs of (y - target)^2 to grad; loss
+ * (device float, may be null) receives L */
+int gqz_mlp_grad_dev(const float *params, const float *x, const float *target, int batch, float *grad,
+                     float *loss);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected at step >= 1); grad must hold
+ * zeros on entry and is left zeroed; asynchronous on the library stream */
+int gqz_mlp_train_step_dev(float *params, float *grad, float *m, float *v, int step, float lr, const float *x,
+                           const float *target, int batch, float *loss);
+/* install (params_host != null) and enable/disable the learned policy; it is used by QZ calls
+ * whose gqz_params.use_policy is non-zero (before any gqz_set_policy callback) */
+int gqz_mlp_policy(const float *params_host, int enable);
 
 /* ---- kernel timing (no counterpart in the reference) -----------------------------------------
  * HIP events recorded on the library stream around every launch of the enabled kernel classes

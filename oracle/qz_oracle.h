@@ -110,6 +110,11 @@ int orc_piparmq_ns(int nh);
 int orc_piparmq_nw(int nh);
 int orc_piparmq_nibble(int nh);
 
+/* perceptron of the learned parameter policy (mlp_oracle.c) */
+int orc_mlp_nparams(void);
+void orc_mlp_forward(const double *P, const double *X, int B, double *Y);
+double orc_mlp_grad(const double *P, const double *X, const double *T, int B, double *G);
+
 #ifdef __cplusplus
 }
 #endif

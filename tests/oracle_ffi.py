@@ -58,6 +58,9 @@ def lib():
         L.orc_invhse3.argtypes = [dp, C.c_int, dp, dp]
         L.orc_larfg.argtypes = [C.c_int, dp, dp, C.c_int, dp]
         L.orc_lartg.argtypes = [C.c_double, C.c_double, dp, dp, dp]
+        L.orc_mlp_forward.argtypes = [dp, dp, C.c_int, dp]
+        L.orc_mlp_grad.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.orc_mlp_grad.restype = C.c_double
         for f in ("orc_piparmq_ns", "orc_piparmq_nw", "orc_piparmq_nibble"):
             getattr(L, f).argtypes = [C.c_int]
             getattr(L, f).restype = C.c_int
@@ -120,3 +123,20 @@ def compres(A, B, S, T, Q, Z):
     A, B, S, T, Q, Z = (fmat(x) for x in (A, B, S, T, Q, Z))
     lib().orc_compres(n, ptr(A), ptr(B), ptr(S), ptr(T), ptr(Q), ptr(Z), n, ptr(out))
     return out
+
+
+def mlp_forward(P, X):
+    """oracle perceptron forward pass (double precision): X (B x 12) -> Y (B x 3)."""
+    P = np.ascontiguousarray(P, dtype=np.float64); X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.zeros((X.shape[0], 3))
+    lib().orc_mlp_forward(ptr(P), ptr(X.reshape(-1)), X.shape[0], ptr(Y.reshape(-1)))
+    return Y
+
+
+def mlp_grad(P, X, T):
+    """oracle loss and gradient: returns (L, dL/dP)."""
+    P = np.ascontiguousarray(P, dtype=np.float64); X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    G = np.zeros(P.size)
+    Lv = lib().orc_mlp_grad(ptr(P), ptr(X.reshape(-1)), ptr(T.reshape(-1)), X.shape[0], ptr(G))
+    return Lv, G

@@ -1,0 +1,113 @@
+"""GPU tests of the learned parameter policy: the FP32 inference and gradient kernels against the
+double-precision oracle (oracle/mlp_oracle.c, itself pinned by finite differences in
+test_oracle.py), a training run of the Adam kernel, and the trained policy against the default
+PIPARMQ parameters on a held-out pencil."""
+import ctypes as C
+import os
+import time
+
+import numpy as np
+import pytest
+
+import qzcheck as QC
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    import torch
+    import gqz
+    import oracle_ffi as O
+    L = gqz.init(0)
+    return torch, gqz, O, L
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def test_mlp_forward_matches_oracle():
+    torch, gqz, O, L = _mods()
+    print("mlp forward", flush=True)
+    rng = np.random.default_rng(1)
+    P = (rng.standard_normal(5187) * 0.3).astype(np.float32)
+    X = rng.standard_normal((300, 12)).astype(np.float32)
+    dP, dX = torch.from_numpy(P).cuda(), torch.from_numpy(X).cuda()
+    dY = torch.zeros((300, 3), dtype=torch.float32, device="cuda")
+    assert L.gqz_mlp_forward_dev(_vp(dP), _vp(dX), 300, _vp(dY)) == 0
+    ref = O.mlp_forward(P.astype(np.float64), X.astype(np.float64))
+    err = np.abs(dY.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < 2e-5, err            # FP32 kernel vs FP64 oracle on the same FP32 inputs
+
+
+@pytest.mark.parametrize("B", [1, 64, 200])
+def test_mlp_gradient_matches_oracle(B):
+    torch, gqz, O, L = _mods()
+    print("mlp gradient", B, flush=True)
+    rng = np.random.default_rng(10 + B)
+    P = (rng.standard_normal(5187) * 0.3).astype(np.float32)
+    X = rng.standard_normal((B, 12)).astype(np.float32)
+    T = rng.standard_normal((B, 3)).astype(np.float32)
+    dP, dX, dT = (torch.from_numpy(a).cuda() for a in (P, X, T))
+    dG = torch.zeros(5187, dtype=torch.float32, device="cuda")
+    dL = torch.zeros(1, dtype=torch.float32, device="cuda")
+    assert L.gqz_mlp_grad_dev(_vp(dP), _vp(dX), _vp(dT), B, _vp(dG), _vp(dL)) == 0
+    Lref, Gref = O.mlp_grad(P.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    g = dG.cpu().numpy().astype(np.float64)
+    assert abs(dL.item() - Lref) <= 1e-5 * Lref
+    assert np.linalg.norm(g - Gref) <= 1e-5 * np.linalg.norm(Gref)
+    # every parameter block individually
+    for a, b in [(0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 5184), (5184, 5187)]:
+        assert np.linalg.norm(g[a:b] - Gref[a:b]) <= 1e-4 * max(np.linalg.norm(Gref[a:b]), 1e-12)
+
+
+def test_mlp_adam_training_fits_a_teacher():
+    """The Adam kernel drives the loss down on a target produced by a random teacher network."""
+    torch, gqz, O, L = _mods()
+    import gqz_policy as GP
+    print("mlp training", flush=True)
+    rng = np.random.default_rng(7)
+    X = rng.uniform(0, 1, size=(2048, 12)).astype(np.float32)
+    teacher = (rng.standard_normal(5187) * 0.3).astype(np.float32)
+    Y = O.mlp_forward(teacher.astype(np.float64), X.astype(np.float64)).astype(np.float32)
+    P, losses = GP.train(L, X, Y, steps=600, lr=3e-3, batch=256, seed=1)
+    # the untrained network outputs exactly zero (last layer initialised to zero)
+    initial = float(np.mean(Y.astype(np.float64) ** 2))
+    assert losses[-1] < 0.05 * initial, (initial, losses)
+    assert losses[-1] < losses[0]
+    pred = O.mlp_forward(P.astype(np.float64), X.astype(np.float64))
+    assert np.mean((pred - Y) ** 2) < 0.1 * np.mean((Y - Y.mean(0)) ** 2)
+
+
+def test_learned_policy_versus_default():
+    """The trained policy on a held-out pencil: same backward-error bar as the default parameters,
+    and the QZ time with the learned NS / NW / NIBBLE is reported next to the default's."""
+    torch, gqz, O, L = _mods()
+    import gqz_policy as GP
+    if not os.path.exists(GP.MODEL):
+        pytest.skip("no trained policy")
+    n = 3000
+    print("learned vs default", n, flush=True)
+    H0, T0 = GP.make_pencil("ht", n, 424242)
+    res = {}
+    for mode in ("default", "learned", "default"):
+        if mode == "learned":
+            GP.install_learned(L)
+        else:
+            GP.disable(L)
+        H = gqz.colmajor(n); T = gqz.colmajor(n); Q = gqz.colmajor(n); Z = gqz.colmajor(n)
+        H.copy_(H0); T.copy_(T0)
+        prm = GP.learned_params() if mode == "learned" else gqz.GqzParams()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, _, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        assert info == 0
+        r = gqz.compres(H0, T0, H, T, Q, Z)
+        assert r[0] < 1e-13 and r[1] < 1e-13 and r[2] < 10 and r[3] < 10
+        res[mode] = min(dt, res.get(mode, 1e30))
+    GP.disable(L)
+    print("held-out n=%d: default %.2f s, learned %.2f s, gain %.2fx"
+          % (n, res["default"], res["learned"], res["default"] / res["learned"]), flush=True)
+    assert res["learned"] <= 1.05 * res["default"]

@@ -222,3 +222,43 @@ def test_oracle_chase_window_equivalence():
     np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-13)
     np.testing.assert_allclose(U.T @ H0 @ V, H, atol=1e-12 * np.abs(H0).max())
     np.testing.assert_allclose(U.T @ T0 @ V, T, atol=1e-12 * np.abs(T0).max())
+
+
+# ---- perceptron of the learned parameter policy ---------------------------------------------------
+def _mlp_case(seed, B=7):
+    rng = np.random.default_rng(seed)
+    P = rng.standard_normal(5187) * 0.2
+    X = rng.standard_normal((B, 12))
+    T = rng.standard_normal((B, 3))
+    return P, X, T
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The oracle's analytic gradient (pinned here) against central differences of its loss, on a
+    sample of parameters from every block (W1, b1, W2, b2, W3, b3)."""
+    P, X, T = _mlp_case(3)
+    L0, G = O.mlp_grad(P, X, T)
+    loss = lambda Q: float(np.mean((O.mlp_forward(Q, X) - T) ** 2))
+    assert abs(L0 - loss(P)) < 1e-12 * max(1.0, L0)
+    rng = np.random.default_rng(4)
+    blocks = [(0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 5184), (5184, 5187)]
+    idx = np.concatenate([rng.integers(a, b, size=min(6, b - a)) for a, b in blocks])
+    h = 1e-6
+    for i in idx:
+        Pp = P.copy(); Pp[i] += h
+        Pm = P.copy(); Pm[i] -= h
+        fd = (loss(Pp) - loss(Pm)) / (2 * h)
+        assert abs(fd - G[i]) <= 1e-6 * max(1.0, abs(G[i])), (i, fd, G[i])
+
+
+def test_mlp_oracle_forward_closed_form():
+    """With W2 = 0 the network output is W3 gelu(b2) + b3 for every input (a closed form)."""
+    P, X, _ = _mlp_case(5)
+    P[832:4928] = 0.0
+    from math import erf, sqrt
+    gelu = lambda z: 0.5 * z * (1 + erf(z / sqrt(2)))
+    a2 = np.array([gelu(z) for z in P[4928:4992]])
+    W3 = P[4992:5184].reshape(3, 64)
+    expect = W3 @ a2 + P[5184:5187]
+    Y = O.mlp_forward(P, X)
+    assert np.allclose(Y, expect[None, :], rtol=0, atol=1e-13)
