@@ -57,6 +57,62 @@ def ht_random_device(n, seed, torch, gqz):
     return H, T
 
 
+def batched_workload(args, torch, gqz, L, log):
+    """4096 independent N(0,1) pencils with n = round(32 * 32^u), u ~ U(0,1) (log-uniform in
+    [32, 1024]): QR(B) + HT reduction + QZ with Q and Z, one persistent launch replayed as a hipGraph,
+    launch shape from the autotuner.  A step is one pass over the whole batch."""
+    import ctypes as C
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    count = args.batch
+    ns = np.round(32.0 * 32.0 ** rng.uniform(0, 1, count)).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]]).astype(np.int64)
+    tot = int((ns.astype(np.int64) ** 2).sum()); et = int(ns.sum())
+    g = torch.Generator(device="cuda"); g.manual_seed(args.seed)
+    A0 = torch.randn(tot, dtype=torch.float64, device="cuda", generator=g)
+    B0 = torch.randn(tot, dtype=torch.float64, device="cuda", generator=g)
+    A = torch.empty_like(A0); Bm = torch.empty_like(B0); Q = torch.empty_like(A0); Z = torch.empty_like(A0)
+    ar = torch.empty(et, dtype=torch.float64, device="cuda"); ai = torch.empty_like(ar); be = torch.empty_like(ar)
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    bt = C.c_int(0); bw = C.c_int(0); bms = C.c_double(0)
+    t0 = time.perf_counter()
+    L.gqz_batched_tune(count, ns.ctypes.data_as(C.c_void_p), C.byref(bt), C.byref(bw), C.byref(bms))
+    log("autotuner: %d threads, %d groups per CU budget (%.1f ms on the sample) in %.1f s"
+        % (bt.value, bw.value, bms.value, time.perf_counter() - t0))
+    args_c = lambda: (count, ns.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                      eoff.ctypes.data_as(C.c_void_p), vp(A), vp(Bm), vp(Q), vp(Z), vp(ar), vp(ai), vp(be),
+                      vp(info), 0)
+    for w in range(args.warmup):
+        A.copy_(A0); Bm.copy_(B0)
+        torch.cuda.synchronize()
+        assert L.gqz_batched_dev(*args_c()) == 0
+    times = []
+    for s in range(args.steps):
+        A.copy_(A0); Bm.copy_(B0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r = L.gqz_batched_dev(*args_c())
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        assert r == 0, L.gqz_last_error()
+        log("batched step %d: %.3f s" % (s, times[-1]))
+    # check a sample of the pencils (largest, smallest, random) against their inputs
+    worst = [0.0, 0.0, 0.0, 0.0]
+    pick = list(np.argsort(ns)[-3:]) + list(np.argsort(ns)[:3]) + list(rng.choice(count, 6, replace=False))
+    for i in pick:
+        k = int(ns[i]); o = int(off[i])
+        M = lambda t: t[o:o + k * k].reshape(k, k).t()     # column-major view
+        res = gqz.compres(M(A0), M(B0), M(A), M(Bm), M(Q), M(Z))
+        worst = [max(a, b) for a, b in zip(worst, res)]
+    flops = float((66.0 * ns.astype(np.float64) ** 3).sum())
+    return times, {"count": count, "n_min": int(ns.min()), "n_max": int(ns.max()), "sum_n3": float((ns.astype(np.float64) ** 3).sum()),
+                   "threads": bt.value, "info_nonzero": int((info != 0).sum().item()),
+                   "worst_resA": worst[0], "worst_resB": worst[1], "worst_resQ_neps": worst[2],
+                   "worst_resZ_neps": worst[3], "gflops_66n3": flops / min(times) / 1e9}
+
+
 def cpu_baseline(n_target, n_sample, seed):
     """Oracle QZ (the C restatement, one core) on an n_sample HT-random pencil; cubic extrapolation."""
     import numpy as np
@@ -84,6 +140,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--ktime", type=int, default=1, help="record per-class kernel event times")
+    ap.add_argument("--workload", choices=["qz", "batched"], default="qz")
+    ap.add_argument("--batch", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this QZ build has no multi-GPU path (see the sharding "
@@ -104,6 +162,19 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.workload == "batched":
+        times, info = batched_workload(args, torch, gqz, L, log)
+        out = {"metric": "batched pencils/s (QR + HT + QZ with Q, Z; %d pencils, n log-uniform in [32, 1024])"
+                         % info["count"],
+               "value": info["count"] / min(times), "unit": "pencils/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": 1000.0 * sum(times) / len(times), "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic: A, B i.i.d. N(0,1) generated on the device, seed %d" % args.seed,
+               "config": {"workload": "batched QZ of %d pencils" % info["count"], **info},
+               "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
 
     # warmup steps (same pipeline, smaller pencil: loads code objects, sizes the workspace)
     for w in range(args.warmup):

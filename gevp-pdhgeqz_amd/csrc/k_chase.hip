@@ -111,8 +111,9 @@ __global__ void __launch_bounds__(256) k_chase(GqzChainSet S) {
         }
         __syncthreads();
         // apply all left transforms: per bulge 3w items (H cols, T cols, U rows)
-        for (int idx = tid; idx < nb * 3 * w; idx += blockDim.x) {
-            const int b = idx / (3 * w), r = idx % (3 * w);
+        for (int idx = tid, b = tid / (3 * w), r = tid - (tid / (3 * w)) * (3 * w); idx < nb * 3 * w;
+             idx += blockDim.x, r += blockDim.x) {
+            while (r >= 3 * w) { r -= 3 * w; ++b; }     // (b, r) = (idx / 3w, idx % 3w) without dividing
             const int kind = bk[b];
             if (kind == 0) continue;
             const int c = bc[b];
@@ -174,8 +175,9 @@ __global__ void __launch_bounds__(256) k_chase(GqzChainSet S) {
             }
         }
         __syncthreads();
-        for (int idx = tid; idx < nb * 3 * w; idx += blockDim.x) {
-            const int b = idx / (3 * w), r = idx % (3 * w);
+        for (int idx = tid, b = tid / (3 * w), r = tid - (tid / (3 * w)) * (3 * w); idx < nb * 3 * w;
+             idx += blockDim.x, r += blockDim.x) {
+            while (r >= 3 * w) { r -= 3 * w; ++b; }
             const int kind = bk[b];
             if (kind == 0) continue;
             const double *B = bv + 8 * b;
