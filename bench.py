@@ -77,10 +77,12 @@ def batched_workload(args, torch, gqz, L, log):
     info = torch.zeros(count, dtype=torch.int32, device="cuda")
     vp = lambda t: C.c_void_p(t.data_ptr())
     bt = C.c_int(0); bw = C.c_int(0); bms = C.c_double(0)
-    t0 = time.perf_counter()
-    L.gqz_batched_tune(count, ns.ctypes.data_as(C.c_void_p), C.byref(bt), C.byref(bw), C.byref(bms))
-    log("autotuner: %d threads, %d groups per CU budget (%.1f ms on the sample) in %.1f s"
-        % (bt.value, bw.value, bms.value, time.perf_counter() - t0))
+    if args.tune:
+        log("autotuner on a sample of the size distribution ...")
+        t0 = time.perf_counter()
+        L.gqz_batched_tune(count, ns.ctypes.data_as(C.c_void_p), C.byref(bt), C.byref(bw), C.byref(bms))
+        log("autotuner: %d threads, %d workgroups (%.1f ms on the sample) in %.1f s"
+            % (bt.value, bw.value, bms.value, time.perf_counter() - t0))
     args_c = lambda: (count, ns.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
                       eoff.ctypes.data_as(C.c_void_p), vp(A), vp(Bm), vp(Q), vp(Z), vp(ar), vp(ai), vp(be),
                       vp(info), 0)
@@ -142,6 +144,7 @@ def main():
     ap.add_argument("--ktime", type=int, default=1, help="record per-class kernel event times")
     ap.add_argument("--workload", choices=["qz", "batched"], default="qz")
     ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--tune", type=int, default=1, help="run the batched autotuner first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this QZ build has no multi-GPU path (see the sharding "
@@ -155,10 +158,10 @@ def main():
     L.gqz_ktime_get.argtypes = [C.POINTER(KTime), C.c_int]
     n = args.n
     prm = gqz.GqzParams()
-    policy = None
     if args.params == "learned":
         import gqz_policy
-        policy = gqz_policy.install_learned(L)
+        gqz_policy.install_learned(L)
+        prm = gqz_policy.learned_params()
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)

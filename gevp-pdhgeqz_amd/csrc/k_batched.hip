@@ -253,15 +253,16 @@ int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, 
     try {
         if (gqz_init_current()) return -100;
         hipStream_t st = (hipStream_t)gqz_stream();
-        // sample: up to 256 pencils at the quantiles of the requested size distribution
+        // sample: up to 64 pencils at the quantiles of the requested size distribution, orders capped
+        // at 512 so that one candidate stays short
         std::vector<int> s(n, n + count);
         std::sort(s.begin(), s.end());
-        const int ns = std::min(count, 256);
+        const int ns = std::min(count, 64);
         std::vector<int> sn(ns);
         std::vector<long> so(ns), se(ns);
         long tot = 0, et = 0;
         for (int i = 0; i < ns; ++i) {
-            sn[i] = s[(size_t)i * count / ns];
+            sn[i] = std::min(512, s[(size_t)i * count / ns]);
             so[i] = tot; se[i] = et;
             tot += (long)sn[i] * sn[i]; et += sn[i];
         }
@@ -273,8 +274,7 @@ int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, 
                            0x5EEDull);
         hipEvent_t e0, e1;
         GQZ_HIP(hipEventCreate(&e0)); GQZ_HIP(hipEventCreate(&e1));
-        const int cand_nt[4] = { 128, 256, 512, 1024 };
-        const int cand_occ[3] = { 1, 2, 4 };
+        const int cand_nt[3] = { 128, 256, 512 };
         float best = 1e30f;
         int bnt = 256, bnwg = std::min(ns, 512);
         ensure_cap(ns);
@@ -293,21 +293,20 @@ int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, 
         a.count = ns; a.order = B.d_meta; a.n = B.d_meta + ns; a.off = B.d_off; a.eoff = B.d_off + ns;
         a.A = A; a.B = Bm; a.Q = Q; a.Z = Z; a.ar = ar; a.ai = ai; a.be = be; a.info = nullptr;
         a.counter = B.d_meta + 2 * ns;
-        // the sample holds ns pencils; the persistent grid is scaled to the full batch's share
-        for (int ci = 0; ci < 4; ++ci) {
-            for (int oi = 0; oi < 3; ++oi) {
-                int nt = cand_nt[ci];
-                int nwg = std::max(1, std::min(ns, 256 * cand_occ[oi] * ns / std::max(ns, count)));
-                GQZ_HIP(hipMemcpyAsync(A, A0, 2 * (size_t)tot * sizeof(double), hipMemcpyDeviceToDevice, st));
-                GQZ_HIP(hipEventRecord(e0, st));
-                launch(st, a, nt, nwg);
-                GQZ_HIP(hipEventRecord(e1, st));
-                GQZ_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                GQZ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; bnt = nt; bnwg = 256 * cand_occ[oi]; }
-            }
+        // every sample pencil gets its own workgroup, as the leading (largest) pencils of a real batch
+        // do; the workgroup size is what the sample decides, the grid follows from it
+        for (int ci = 0; ci < 3; ++ci) {
+            const int nt = cand_nt[ci];
+            GQZ_HIP(hipMemcpyAsync(A, A0, 2 * (size_t)tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+            GQZ_HIP(hipEventRecord(e0, st));
+            launch(st, a, nt, ns);
+            GQZ_HIP(hipEventRecord(e1, st));
+            GQZ_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            GQZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bnt = nt; }
         }
+        bnwg = 256 * (bnt <= 256 ? 4 : 2);
         GQZ_HIP(hipEventDestroy(e0)); GQZ_HIP(hipEventDestroy(e1));
         GQZ_HIP(hipFree(buf));
         B.tuned_nt = bnt; B.tuned_nwg = bnwg;

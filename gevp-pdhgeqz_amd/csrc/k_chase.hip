@@ -23,8 +23,9 @@
 
 #define CH_MAXW 64
 #define CH_MAXB 21
+#define CH_NT 1024         // 16 waves: the LDS round trips of the update phases overlap
 
-__global__ void __launch_bounds__(256) k_chase(GqzChainSet S) {
+__global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     extern __shared__ double lds[];
     const int cidx = blockIdx.x;
     GqzChainArgs a;
@@ -260,7 +261,7 @@ void gqz_chase_set(hipStream_t st, const GqzChainSet &s) {
         hipFuncSetAttribute((const void *)k_chase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         lds_set = lds;
     }
-    hipLaunchKernelGGL(k_chase, dim3(s.nch), dim3(256), lds, st, s);
+    hipLaunchKernelGGL(k_chase, dim3(s.nch), dim3(CH_NT), lds, st, s);
 }
 
 void gqz_chase_window(hipStream_t st, const GqzChainArgs &a) {

@@ -176,9 +176,10 @@ int gqz_batched_dev(int count, const int *n, const long *off, const long *eoff,
  * wg_threads: 0 = tuned shape (or the size heuristic before tuning), else 128/256/512/1024.
  * Repeated calls with the same arguments replay a captured hipGraph. */
 
-/* Autotuner for the batched launch: times every (workgroup size, groups per CU) candidate on a
+/* Autotuner for the batched launch: times every workgroup size (128, 256, 512) on a
  * synthetic sample drawn at the quantiles of the size list n (host array) and keeps the fastest
- * for later gqz_batched_dev calls with wg_threads = 0. */
+ * for later gqz_batched_dev calls with wg_threads = 0 (sample: up to 64 pencils, orders capped at
+ * 512; the grid then holds 4 (<= 256 threads) or 2 workgroups per CU). */
 int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, double *best_ms);
 
 #ifdef __cplusplus
