@@ -2,14 +2,18 @@
 
 Workload (default): Schur form (JOB='S', COMPQ=COMPZ='I') of an n=16000 Hessenberg-triangular pencil
 drawn as in testing/EXRAND1.f:243-279 (N(0,1) upper parts, chi-distributed subdiagonal of H and
-diagonal of T: the distribution of an HT-reduced N(0,1) pencil), with the PIPARMQ default parameters
-(NS=512, NW=768, NIBBLE=5 at n=16000) or the learned parameter policy (--params learned).
+diagonal of T: the distribution of an HT-reduced N(0,1) pencil), with the learned parameter policy
+(default) or the PIPARMQ default parameters (--params default: NS=512, NW=768, NIBBLE=5 at n=16000).
 A "step" is one complete QZ of that pencil; the pencil is generated on the device and restored from a
-pristine device copy before each step (inputs resident in HBM when timing starts).
+pristine device copy before each step (inputs resident in HBM when timing starts).  After the timed
+steps the result is checked (PCOMPRES residuals) and, with --compare-n m, one pencil of order m is
+solved with both parameter sets for the learned-versus-default comparison.
 
-Output: one JSON line with the QZ time in seconds, the roofline of the dominant kernel class
-(HIP-event time on the library stream, algorithmic bytes or flops of its launches) and a CPU baseline
-(the oracle's QZ on a bounded sample, cubically extrapolated to the workload's n).
+Output: one JSON line with the QZ time in seconds, the roofline of the dominant kernel class and of
+the MFMA slab update (HIP-event time on the library stream, algorithmic bytes or flops of their
+launches), the comparison, and a CPU baseline (the oracle's QZ on a bounded sample, cubically
+extrapolated to the workload's n).
+--workload batched measures the batched mode instead (pencils per second).
 """
 import argparse
 import ctypes as C
@@ -136,7 +140,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--params", choices=["default", "learned"], default="default")
+    ap.add_argument("--params", choices=["default", "learned"], default="learned")
+    ap.add_argument("--compare-n", type=int, default=4000,
+                    help="order of an extra default-vs-learned comparison run (0: none)")
     ap.add_argument("--warmup-n", type=int, default=2000, help="pencil order of the warmup steps")
     ap.add_argument("--cpu-sample-n", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -211,22 +217,31 @@ def main():
     classes = [dict(name=kt[i].name.decode(), launches=kt[i].launches, ms=kt[i].ms, flops=kt[i].flops,
                     bytes=kt[i].bytes) for i in range(max(nk, 0))]
     ms_per_step = 1000.0 * t_all / args.steps
-    roof = None
-    if classes and any(c["launches"] for c in classes):
-        dom = max(classes, key=lambda c: c["ms"])
-        avg_s = dom["ms"] / 1000.0 / dom["launches"]
-        if dom["flops"] > 0 and dom["flops"] / max(dom["bytes"], 1.0) > RIDGE:
-            ach = dom["flops"] / dom["launches"] / avg_s / 1e12
-            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+    def roofline(c):
+        avg_s = c["ms"] / 1000.0 / c["launches"]
+        if c["flops"] > 0 and c["flops"] / max(c["bytes"], 1.0) > RIDGE:
+            ach = c["flops"] / c["launches"] / avg_s / 1e12
+            r = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
         else:
-            ach = dom["bytes"] / dom["launches"] / avg_s / 1e9
-            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-        roof["traffic"] = None
-        roof["kernel"] = dom["name"]
-        roof["avg_launch_us"] = 1e6 * avg_s
-        roof["launches_per_step"] = dom["launches"] / args.steps
-        roof["share_of_step"] = dom["ms"] / 1000.0 / t_all
+            ach = c["bytes"] / c["launches"] / avg_s / 1e9
+            r = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+        r["frac"] = r["achieved"] / r["peak"]
+        r["traffic"] = None
+        r["kernel"] = c["name"]
+        r["avg_launch_us"] = 1e6 * avg_s
+        r["launches_per_step"] = c["launches"] / args.steps
+        r["share_of_step"] = c["ms"] / 1000.0 / t_all
+        if c["flops"] > 0:
+            r["tflops"] = c["flops"] / c["launches"] / avg_s / 1e12
+            r["gbytes_per_s"] = c["bytes"] / c["launches"] / avg_s / 1e9
+        return r
+    roof, roof_slab = None, None
+    live = [c for c in classes if c["launches"]]
+    if live:
+        roof = roofline(max(live, key=lambda c: c["ms"]))
+        slabs = [c for c in live if c["name"] == "slab_update"]
+        if slabs:
+            roof_slab = roofline(slabs[0])
     check = None
     if not args.no_check:
         res = gqz.compres(H0, T0, H, T, Q, Z)
@@ -234,12 +249,37 @@ def main():
         check = {"resA": res[0], "resB": res[1], "resQ_neps": res[2], "resZ_neps": res[3],
                  "n_infinite": int((be_h == 0).sum()), "info": infos[-1]}
         log("check:", check)
+    comparison = None
+    if args.compare_n > 0:
+        # learned versus default parameters on a second pencil of the same family (not timed in value)
+        import gqz_policy
+        m = args.compare_n
+        Hc0, Tc0 = ht_random_device(m, args.seed + 1, torch, gqz)
+        comparison = {"n": m}
+        for mode in ("default", "learned"):
+            if mode == "learned":
+                gqz_policy.install_learned(L)
+                pc = gqz_policy.learned_params()
+            else:
+                pc = gqz.GqzParams()
+            Hc = gqz.colmajor(m); Tc = gqz.colmajor(m); Qc = gqz.colmajor(m); Zc = gqz.colmajor(m)
+            Hc.copy_(Hc0); Tc.copy_(Tc0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, _, infoc, _ = gqz.pdhgeqz("S", "I", "I", Hc, Tc, Qc, Zc, params=pc)
+            torch.cuda.synchronize()
+            comparison[mode + "_s"] = time.perf_counter() - t0
+            rc = gqz.compres(Hc0, Tc0, Hc, Tc, Qc, Zc)
+            comparison[mode + "_resA"] = rc[0]
+            log("comparison n=%d %s: %.3f s info=%d resA=%.2e" % (m, mode, comparison[mode + "_s"], infoc, rc[0]))
+            del Hc, Tc, Qc, Zc
+        comparison["gain"] = comparison["default_s"] / comparison["learned_s"]
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, args.cpu_sample_n, args.seed)
     st = stats[-1]
     out = {
-        "metric": "QZ seconds, n=%d pencil, %s parameters" % (n, args.params),
+        "metric": "QZ seconds, n=%d pencil, %s parameters (bkwd err checked)" % (n, args.params),
         "value": t_all / args.steps,
         "unit": "s",
         "n_gpus": 1,
@@ -256,7 +296,9 @@ def main():
                    "params": args.params, "ns": st.get("shifts_used", 0), "sweeps": st["sweeps"],
                    "aed_calls": st["aed_calls"], "iterations": st["iters"]},
         "roofline": roof,
+        "roofline_slab_update": roof_slab,
         "cpu_baseline": cpu,
+        "learned_vs_default": comparison,
         "check": check,
         "kernel_classes": classes,
         "step_seconds": times,

@@ -77,6 +77,62 @@ __device__ __forceinline__ void d_larfg2(double &alpha, double &x1, double &tau)
     alpha = beta;
 }
 
+// ---- short-latency variants for the one-lane chains of the chase and the window QZ ----------
+// A division compiles to a dozen dependent instructions and hypot to a library sequence; the
+// chains below use one hardware reciprocal refined by two Newton steps (the correctly rounded
+// reciprocal in all but rare cases, error below one ulp otherwise) and plain square roots of
+// sums of squares, with the exact routines outside the range where neither can under- or
+// overflow.  Every reflector keeps tau = 2 / (v^T v) of its own rounded v (one exact division),
+// so that it is orthogonal to working precision whatever the rounding of v.
+
+// 1/x
+__device__ __forceinline__ double d_rcp(double x) {
+    const double ax = fabs(x);
+    if (!(ax > 1e-290 && ax < 1e290)) return 1.0 / x;
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+// DLARFG for [alpha; x1; x2] (the d_larfg3 reflector: v = [1; x / (alpha - beta)])
+__device__ __forceinline__ void d_larfg3f(double &alpha, double &x1, double &x2, double &tau) {
+    const double m = fmax(fabs(alpha), fmax(fabs(x1), fabs(x2)));
+    if (!(m > 1e-140 && m < 1e140)) { d_larfg3(alpha, x1, x2, tau); return; }
+    const double s = fma(x1, x1, x2 * x2);
+    if (s == 0.0) { tau = 0.0; return; }
+    const double beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+    const double rd = d_rcp(alpha - beta);
+    x1 *= rd; x2 *= rd;
+    tau = 2.0 / fma(x1, x1, fma(x2, x2, 1.0));
+    alpha = beta;
+}
+
+// DLARFG for [alpha; x1]; tau from the fast reciprocal (for intermediate reflectors whose
+// orthogonality only enters through a vector that is reflected again)
+__device__ __forceinline__ void d_larfg2f(double &alpha, double &x1, double &tau) {
+    if (x1 == 0.0) { tau = 0.0; return; }
+    const double m = fmax(fabs(alpha), fabs(x1));
+    if (!(m > 1e-140 && m < 1e140)) { d_larfg2(alpha, x1, tau); return; }
+    const double beta = -copysign(sqrt(fma(alpha, alpha, x1 * x1)), alpha);
+    x1 *= d_rcp(alpha - beta);
+    tau = 2.0 * d_rcp(fma(x1, x1, 1.0));
+    alpha = beta;
+}
+
+// DLARTG: c = |f| / d, s = sign(f) g / d, r = sign(f) d with d = sqrt(f^2 + g^2)
+__device__ __forceinline__ void d_lartgf(double f, double g, double &c, double &s, double &r) {
+    if (g == 0.0) { c = 1.0; s = 0.0; r = f; return; }
+    if (f == 0.0) { c = 0.0; s = (g > 0.0) ? 1.0 : -1.0; r = fabs(g); return; }
+    const double m = fmax(fabs(f), fabs(g));
+    if (!(m > 1e-140 && m < 1e140)) { d_lartg(f, g, c, s, r); return; }
+    const double d = sqrt(fma(f, f, g * g));
+    const double rd = d_rcp(d);
+    c = fabs(f) * rd;
+    s = (f > 0.0 ? g : -g) * rd;
+    r = copysign(d, f);
+}
+
 // Eigenvalues of a 2x2 pencil (A, B), B upper triangular with non-zero diagonal:
 // lambda_k = (ar_k + i ai_k), b = 1.  Standard 2x2 formula on M = B^{-1} A.
 __device__ inline void d_eig22(double a00, double a10, double a01, double a11,
@@ -154,6 +210,27 @@ __device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &
     double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
     double p1 = w1 / p0, p2 = w2 / p0;
     tau = 2.0 / (1.0 + p1 * p1 + p2 * p2);
+    v[0] = 1.0; v[1] = p1; v[2] = p2;
+}
+
+// d_invhse3 from rows 1 and 2 of the block only (the same two reflectors and the same final
+// reflector), with the short-latency primitives
+__device__ __forceinline__ void d_invhse3f(double r10, double r11, double r12, double r20, double r21, double r22,
+                                           double v[3], double &tau) {
+    double a = r22, x0 = r20, x1 = r21, tau1;
+    d_larfg3f(a, x0, x1, tau1);                         // u1 = (x0, x1, 1) on columns 0, 1, 2
+    const double s = tau1 * (r10 * x0 + r11 * x1 + r12);
+    double a2 = r11 - s * x1, y = r10 - s * x0, tau2;  // row 1 after the first reflector
+    d_larfg2f(a2, y, tau2);                             // u2 = (y, 1) on columns 0, 1
+    const double s2 = tau2 * y;
+    double w0 = 1.0 - s2 * y, w1 = -s2;
+    const double s1 = tau1 * (w0 * x0 + w1 * x1);
+    w0 -= s1 * x0; w1 -= s1 * x1;
+    const double w2 = -s1;
+    const double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
+    const double rp = d_rcp(p0);
+    const double p1 = w1 * rp, p2 = w2 * rp;
+    tau = 2.0 / fma(p1, p1, fma(p2, p2, 1.0));
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 

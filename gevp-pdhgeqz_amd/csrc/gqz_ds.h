@@ -17,49 +17,49 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
         if (tid == 0) {
             double *H = P.H, *T = P.T;
             int ld = P.ld;
-            double v0, v1, v2, tau;
-            if (k == f) {
-                double v[3];
-                d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
-                            C->sb[0], C->sr[1], C->sb[1], v);
-                v0 = v[0]; v1 = v[1]; v2 = v[2];
-                d_larfg3(v0, v1, v2, tau);
-            } else {
-                T[GQZ_IX(k, k - 1, ld)] = 0.0; T[GQZ_IX(k + 1, k - 1, ld)] = 0.0;
-                v0 = H[GQZ_IX(k, k - 1, ld)]; v1 = H[GQZ_IX(k + 1, k - 1, ld)]; v2 = H[GQZ_IX(k + 2, k - 1, ld)];
-                double beta = v0;
-                d_larfg3(beta, v1, v2, tau);
-                if (H[GQZ_IX(k + 2, k - 1, ld)] != 0.0 || H[GQZ_IX(k + 2, k, ld)] != 0.0 ||
-                    H[GQZ_IX(k + 2, k + 1, ld)] == 0.0) {
-                    H[GQZ_IX(k, k - 1, ld)] = beta;
-                } else {
-                    // collapsed bulge: try to re-introduce it (DHGEQZ5.f:141-189)
-                    double w[3];
-                    d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
-                                C->sb[0], C->sr[1], C->sb[1], w);
-                    double a1 = w[0], x1 = w[1], x2 = w[2], tau1;
-                    d_larfg3(a1, x1, x2, tau1);
-                    double refsum = tau1 * (H[GQZ_IX(k, k - 1, ld)] + x1 * H[GQZ_IX(k + 1, k - 1, ld)]);
-                    if (fabs(H[GQZ_IX(k + 1, k - 1, ld)] - refsum * x1) + fabs(refsum * x2) >
-                        GQZ_ULP * (fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]) +
-                                   fabs(H[GQZ_IX(k + 1, k + 1, ld)]))) {
-                        H[GQZ_IX(k, k - 1, ld)] = beta;
-                    } else {
-                        H[GQZ_IX(k, k - 1, ld)] -= refsum;
-                        tau = tau1; v1 = x1; v2 = x2;
-                    }
-                }
-                H[GQZ_IX(k + 1, k - 1, ld)] = 0.0; H[GQZ_IX(k + 2, k - 1, ld)] = 0.0;
-            }
-            // the right (opposite) reflector is computed from the 3x3 block of T after the left
-            // reflector, which lane 0 forms privately; lane 0 also updates the 3x3 corners of H and
-            // T (left then right), so that all other entries can be updated in a single phase
+            // all reads first (the writes below touch column k-1 only), then the arithmetic
             double t3[9], h3[9];
             for (int jj = 0; jj < 3; ++jj)
                 for (int ii = 0; ii < 3; ++ii) {
                     t3[ii + 3 * jj] = T[GQZ_IX(k + ii, k + jj, ld)];
                     h3[ii + 3 * jj] = H[GQZ_IX(k + ii, k + jj, ld)];
                 }
+            double v0, v1, v2, tau;
+            if (k == f) {
+                double v[3];
+                d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
+                            C->sb[0], C->sr[1], C->sb[1], v);
+                v0 = v[0]; v1 = v[1]; v2 = v[2];
+                d_larfg3f(v0, v1, v2, tau);
+            } else {
+                const double hp = H[GQZ_IX(k - 1, k - 1, ld)];
+                const double c0 = H[GQZ_IX(k, k - 1, ld)], c1 = H[GQZ_IX(k + 1, k - 1, ld)];
+                const double c2 = H[GQZ_IX(k + 2, k - 1, ld)];
+                T[GQZ_IX(k, k - 1, ld)] = 0.0; T[GQZ_IX(k + 1, k - 1, ld)] = 0.0;
+                v1 = c1; v2 = c2;
+                double beta = c0;
+                d_larfg3f(beta, v1, v2, tau);
+                double hk = beta;
+                if (c2 == 0.0 && h3[2] == 0.0 && h3[5] != 0.0) {
+                    // collapsed bulge: try to re-introduce it (DHGEQZ5.f:141-189)
+                    double w[3];
+                    d_shift_col(H + GQZ_IX(k, k, ld), ld, T + GQZ_IX(k, k, ld), ld, C->sr[0], C->si[0],
+                                C->sb[0], C->sr[1], C->sb[1], w);
+                    double a1 = w[0], x1 = w[1], x2 = w[2], tau1;
+                    d_larfg3f(a1, x1, x2, tau1);
+                    double refsum = tau1 * (c0 + x1 * c1);
+                    if (!(fabs(c1 - refsum * x1) + fabs(refsum * x2) >
+                          GQZ_ULP * (fabs(hp) + fabs(h3[0]) + fabs(h3[4])))) {
+                        hk = c0 - refsum;
+                        tau = tau1; v1 = x1; v2 = x2;
+                    }
+                }
+                H[GQZ_IX(k, k - 1, ld)] = hk;
+                H[GQZ_IX(k + 1, k - 1, ld)] = 0.0; H[GQZ_IX(k + 2, k - 1, ld)] = 0.0;
+            }
+            // the right (opposite) reflector is computed from the 3x3 block of T after the left
+            // reflector, which lane 0 forms privately; lane 0 also updates the 3x3 corners of H and
+            // T (left then right), so that all other entries can be updated in a single phase
             if (tau != 0.0) {
                 for (int jj = 0; jj < 3; ++jj) {
                     double *tc = t3 + 3 * jj, *hc = h3 + 3 * jj;
@@ -70,7 +70,7 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
                 }
             }
             double w[3], tau2;
-            d_invhse3(t3, 3, w, tau2);
+            d_invhse3f(t3[1], t3[4], t3[7], t3[2], t3[5], t3[8], w, tau2);
             if (tau2 != 0.0) {
                 for (int ii = 0; ii < 3; ++ii) {
                     double a = tau2 * (t3[ii] + w[1] * t3[ii + 3] + w[2] * t3[ii + 6]);
@@ -138,7 +138,7 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
         int ld = P.ld;
         T[GQZ_IX(l - 1, l - 2, ld)] = 0.0; T[GQZ_IX(l, l - 2, ld)] = 0.0;
         double c, s, r;
-        d_lartg(H[GQZ_IX(l - 1, l - 2, ld)], H[GQZ_IX(l, l - 2, ld)], c, s, r);
+        d_lartgf(H[GQZ_IX(l - 1, l - 2, ld)], H[GQZ_IX(l, l - 2, ld)], c, s, r);
         H[GQZ_IX(l - 1, l - 2, ld)] = r; H[GQZ_IX(l, l - 2, ld)] = 0.0;
         C->c = c; C->s = s;
     }
@@ -149,7 +149,7 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
         double *T = P.T;
         int ld = P.ld;
         double c, s, r;
-        d_lartg(T[GQZ_IX(l, l, ld)], T[GQZ_IX(l, l - 1, ld)], c, s, r);
+        d_lartgf(T[GQZ_IX(l, l, ld)], T[GQZ_IX(l, l - 1, ld)], c, s, r);
         T[GQZ_IX(l, l, ld)] = r; T[GQZ_IX(l, l - 1, ld)] = 0.0;
         C->c = c; C->s = s;
     }
@@ -222,7 +222,7 @@ static __device__ void ds_decide(const WgPencil &P, WgCtl *C, double *ar, double
         }
         double c, s, r;
         if (T[GQZ_IX(il, il, ld)] == 0.0) {
-            d_lartg(H[GQZ_IX(il, il, ld)], H[GQZ_IX(il, il - 1, ld)], c, s, r);
+            d_lartgf(H[GQZ_IX(il, il, ld)], H[GQZ_IX(il, il - 1, ld)], c, s, r);
             H[GQZ_IX(il, il, ld)] = r; H[GQZ_IX(il, il - 1, ld)] = 0.0;
             ar[il] = r; ai[il] = 0.0; be[il] = 0.0;
             C->op = OP_COLROT; C->i0 = il; C->i1 = il - 1; C->i2 = il; C->i3 = il; C->c = c; C->s = s;
@@ -230,7 +230,7 @@ static __device__ void ds_decide(const WgPencil &P, WgCtl *C, double *ar, double
         }
         // top deflation splits the segment; with the split stack full the zero is chased down instead
         if (T[GQZ_IX(f, f, ld)] == 0.0 && (f == lo || C->sp < 96)) {
-            d_lartg(H[GQZ_IX(f, f, ld)], H[GQZ_IX(f + 1, f, ld)], c, s, r);
+            d_lartgf(H[GQZ_IX(f, f, ld)], H[GQZ_IX(f + 1, f, ld)], c, s, r);
             H[GQZ_IX(f, f, ld)] = r; H[GQZ_IX(f + 1, f, ld)] = 0.0;
             ar[f] = r; ai[f] = 0.0; be[f] = 0.0;
             C->op = OP_ROWROT; C->i0 = f; C->i1 = f + 1; C->i2 = f + 1; C->c = c; C->s = s;
@@ -285,7 +285,7 @@ static __device__ void wg_zchase(const WgPencil &P, WgCtl *C, int j, int kt, int
     for (int jch = j; jch < kb; ++jch) {
         if (tid == 0) {
             double c, s, r;
-            d_lartg(P.T[GQZ_IX(jch, jch + 1, P.ld)], P.T[GQZ_IX(jch + 1, jch + 1, P.ld)], c, s, r);
+            d_lartgf(P.T[GQZ_IX(jch, jch + 1, P.ld)], P.T[GQZ_IX(jch + 1, jch + 1, P.ld)], c, s, r);
             P.T[GQZ_IX(jch, jch + 1, P.ld)] = r; P.T[GQZ_IX(jch + 1, jch + 1, P.ld)] = 0.0;
             C->c = c; C->s = s;
         }
@@ -295,7 +295,7 @@ static __device__ void wg_zchase(const WgPencil &P, WgCtl *C, int j, int kt, int
         if (jch > kt) {
             if (tid == 0) {
                 double c, s, r;
-                d_lartg(P.H[GQZ_IX(jch + 1, jch, P.ld)], P.H[GQZ_IX(jch + 1, jch - 1, P.ld)], c, s, r);
+                d_lartgf(P.H[GQZ_IX(jch + 1, jch, P.ld)], P.H[GQZ_IX(jch + 1, jch - 1, P.ld)], c, s, r);
                 P.H[GQZ_IX(jch + 1, jch, P.ld)] = r; P.H[GQZ_IX(jch + 1, jch - 1, P.ld)] = 0.0;
                 C->c = c; C->s = s;
             }
@@ -381,7 +381,7 @@ static __device__ void wg_gghrd(double *A, double *B, int ld, int n, int ilo, in
         for (int i = ihi; i >= j + 2; --i) {
             if (tid == 0) {
                 double c, s, r, cc, ss;
-                d_lartg(A[GQZ_IX(i - 1, j, ld)], A[GQZ_IX(i, j, ld)], c, s, r);
+                d_lartgf(A[GQZ_IX(i - 1, j, ld)], A[GQZ_IX(i, j, ld)], c, s, r);
                 A[GQZ_IX(i - 1, j, ld)] = r; A[GQZ_IX(i, j, ld)] = 0.0;
                 // 2x2 corners (rows i-1,i x cols i-1,i) of A and B: row rotation first
                 double a00 = A[GQZ_IX(i - 1, i - 1, ld)], a01 = A[GQZ_IX(i - 1, i, ld)];
@@ -393,7 +393,7 @@ static __device__ void wg_gghrd(double *A, double *B, int ld, int n, int ilo, in
                 double nb00 = c * b00, nb10 = -s * b00;
                 double nb01 = c * b01 + s * b11, nb11 = c * b11 - s * b01;
                 // column rotation (i, i-1) zeroing B(i, i-1)
-                d_lartg(nb11, nb10, cc, ss, r);
+                d_lartgf(nb11, nb10, cc, ss, r);
                 // x = col i, y = col i-1: x' = cc x + ss y ; y' = cc y - ss x
                 double xa0 = na01, ya0 = na00, xa1 = na11, ya1 = na10;
                 A[GQZ_IX(i - 1, i, ld)] = cc * xa0 + ss * ya0; A[GQZ_IX(i - 1, i - 1, ld)] = cc * ya0 - ss * xa0;

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                 double v[3];
                 d_shift_col(sH, ld, sT, ld, a.sr[sp], a.si[sp], a.sb[sp], a.sr[sp + 1], a.sb[sp + 1], v);
                 double al = v[0], x1 = v[1], x2 = v[2], tau;
-                d_larfg3(al, x1, x2, tau);
+                d_larfg3f(al, x1, x2, tau);
                 B[0] = x1; B[1] = x2; B[2] = tau;
             } else if (kind == 1) {
                 const int k = c + 1;
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                 if (q > 0) { sT[k + ld * (k - 1)] = 0.0; sT[(k + 1) + ld * (k - 1)] = 0.0; }
                 double h0 = sH[k + ld * c], h1 = sH[(k + 1) + ld * c], h2 = sH[(k + 2) + ld * c];
                 double beta = h0, x1 = h1, x2 = h2, tau;
-                d_larfg3(beta, x1, x2, tau);
+                d_larfg3f(beta, x1, x2, tau);
                 if (h2 != 0.0 || sH[(k + 2) + ld * k] != 0.0 || sH[(k + 2) + ld * (k + 1)] == 0.0) {
                     sH[k + ld * c] = beta;
                 } else {
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                     d_shift_col(sH + k + ld * k, ld, sT + k + ld * k, ld, a.sr[sp], a.si[sp], a.sb[sp],
                                 a.sr[sp + 1], a.sb[sp + 1], wv);
                     double a1 = wv[0], y1 = wv[1], y2 = wv[2], tau1;
-                    d_larfg3(a1, y1, y2, tau1);
+                    d_larfg3f(a1, y1, y2, tau1);
                     double refsum = tau1 * (h0 + y1 * h1);
                     double tst = fabs(sH[c + ld * c]) + fabs(sH[k + ld * k]) + fabs(sH[(k + 1) + ld * (k + 1)]);
                     if (fabs(h1 - refsum * y1) + fabs(refsum * y2) > GQZ_ULP * tst) {
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                 const int l = w - 1;
                 if (q > 0) { sT[(l - 1) + ld * (l - 2)] = 0.0; sT[l + ld * (l - 2)] = 0.0; }
                 double cs, sn, r;
-                d_lartg(sH[(l - 1) + ld * (l - 2)], sH[l + ld * (l - 2)], cs, sn, r);
+                d_lartgf(sH[(l - 1) + ld * (l - 2)], sH[l + ld * (l - 2)], cs, sn, r);
                 sH[(l - 1) + ld * (l - 2)] = r; sH[l + ld * (l - 2)] = 0.0;
                 B[6] = cs; B[7] = sn;
             }
@@ -165,12 +165,13 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
             if (kind == 1 || kind == 3) {
                 const int k = bc[b] + 1;
                 double wv[3], tau2;
-                d_invhse3(sT + k + ld * k, ld, wv, tau2);
+                const double *t = sT + k + ld * k;
+                d_invhse3f(t[1], t[1 + ld], t[1 + 2 * ld], t[2], t[2 + ld], t[2 + 2 * ld], wv, tau2);
                 B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
             } else if (kind == 2) {
                 const int l = w - 1;
                 double cs, sn, r;
-                d_lartg(sT[l + ld * l], sT[l + ld * (l - 1)], cs, sn, r);
+                d_lartgf(sT[l + ld * l], sT[l + ld * (l - 1)], cs, sn, r);
                 sT[l + ld * l] = r; sT[l + ld * (l - 1)] = 0.0;
                 B[6] = cs; B[7] = sn;
             }
