@@ -151,6 +151,9 @@ def main():
     ap.add_argument("--workload", choices=["qz", "batched"], default="qz")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tune", type=int, default=1, help="run the batched autotuner first")
+    ap.add_argument("--calib-mb", type=int, default=0,
+                    help="first fill, then sum, a device buffer of this many MiB (kernels of known byte "
+                         "counts, for scaling the hardware FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this QZ build has no multi-GPU path (see the sharding "
@@ -185,6 +188,13 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    if args.calib_mb > 0:
+        cb = torch.empty(args.calib_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        cb.fill_(1.0)
+        tot = float(cb.sum())
+        torch.cuda.synchronize()
+        log("calibration: filled and summed %d MiB (sum %.0f)" % (args.calib_mb, tot))
+        del cb
     # warmup steps (same pipeline, smaller pencil: loads code objects, sizes the workspace)
     for w in range(args.warmup):
         Hw, Tw = ht_random_device(args.warmup_n, 1000 + w, torch, gqz)

@@ -234,6 +234,29 @@ __device__ __forceinline__ void d_invhse3f(double r10, double r11, double r12, d
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
+// The same right reflector from the cross product w = r1 x r2 of rows 1 and 2 (w spans their
+// common null space), reflected onto -sign(w0) |w| e0.  The computed w carries an error of order
+// ulp |r1| |r2|, i.e. a residual r_i . w / |w| of order ulp |r_i| / sin(r1, r2); the cross product
+// is used where the rows are at least 1/16 (in sine) from parallel, d_invhse3f elsewhere.
+__device__ __forceinline__ void d_nullrefl3(double r10, double r11, double r12, double r20, double r21, double r22,
+                                            double v[3], double &tau) {
+    const double w0 = fma(r11, r22, -r12 * r21);
+    const double w1 = fma(r12, r20, -r10 * r22);
+    const double w2 = fma(r10, r21, -r11 * r20);
+    const double n1 = fma(r10, r10, fma(r11, r11, r12 * r12));
+    const double n2 = fma(r20, r20, fma(r21, r21, r22 * r22));
+    const double nw = fma(w0, w0, fma(w1, w1, w2 * w2));
+    if (!(fmin(n1, n2) > 1e-130 && fmax(n1, n2) < 1e130 && nw >= 0.00390625 * (n1 * n2))) {
+        d_invhse3f(r10, r11, r12, r20, r21, r22, v, tau);
+        return;
+    }
+    const double p0 = w0 + copysign(sqrt(nw), w0);
+    const double rp = d_rcp(p0);
+    const double p1 = w1 * rp, p2 = w2 * rp;
+    tau = 2.0 / fma(p1, p1, fma(p2, p2, 1.0));
+    v[0] = 1.0; v[1] = p1; v[2] = p2;
+}
+
 // small dense helpers on column-major m x m (m <= 4) arrays
 __device__ inline void d_mm(int m, const double *A, int ta, const double *B, int tb, double *C) {
     for (int j = 0; j < m; ++j)

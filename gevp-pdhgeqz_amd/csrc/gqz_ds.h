@@ -70,7 +70,7 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
                 }
             }
             double w[3], tau2;
-            d_invhse3f(t3[1], t3[4], t3[7], t3[2], t3[5], t3[8], w, tau2);
+            d_nullrefl3(t3[1], t3[4], t3[7], t3[2], t3[5], t3[8], w, tau2);
             if (tau2 != 0.0) {
                 for (int ii = 0; ii < 3; ++ii) {
                     double a = tau2 * (t3[ii] + w[1] * t3[ii + 3] + w[2] * t3[ii + 6]);

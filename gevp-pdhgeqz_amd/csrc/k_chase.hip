@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                 const int k = bc[b] + 1;
                 double wv[3], tau2;
                 const double *t = sT + k + ld * k;
-                d_invhse3f(t[1], t[1 + ld], t[1 + 2 * ld], t[2], t[2 + ld], t[2 + 2 * ld], wv, tau2);
+                d_nullrefl3(t[1], t[1 + ld], t[1 + 2 * ld], t[2], t[2 + ld], t[2 + 2 * ld], wv, tau2);
                 B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
             } else if (kind == 2) {
                 const int l = w - 1;

@@ -273,7 +273,7 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         gqz_launch_aed_small(L.st, v.H, v.T, v.ld, kwtop, jw, kwtop > ktop ? 1 : 0, R.ttol,
                              kSafmin * ((double)jw / kUlp), U, V, weig, weig + jw, weig + 2 * jw, pscr, scr,
                              L.d_out + 16);
-        kt_end(kk, 0.0, 64.0 * jw * jw);
+        kt_end(kk, 0.0, 48.0 * jw * jw);
         HIPCHK(hipMemcpyAsync(L.h_out + 16, L.d_out + 16, 5 * sizeof(int), hipMemcpyDeviceToHost, L.st));
         HIPCHK(hipMemcpyAsync(L.h_buf, weig, 3 * sizeof(double) * jw, hipMemcpyDeviceToHost, L.st));
         HIPCHK(hipStreamSynchronize(L.st));
@@ -314,7 +314,7 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         J.ar = W.ar; J.ai = W.ai; J.be = W.be; J.info = L.d_out + 9; J.ttol = R.ttol; J.smlnum = W.smlnum;
         int kk = kt_begin(GQZ_KC_SMALL);
         gqz_launch_small1(L.st, J);
-        kt_end(kk, 0.0, 64.0 * jw * jw);
+        kt_end(kk, 0.0, 48.0 * jw * jw);
         sync_read(L.h_out + 9, L.d_out + 9, sizeof(int));
         info = L.h_out[9];
     } else {
