@@ -6,11 +6,11 @@
 // relative): its next left reflector acts on rows c+1..c+3 and is computed from H(c+1:c+3, c)
 // (c = -1 means introduction from the shift polynomial).  Bulges are 3 columns apart.
 //
-// Every chase step advances ALL active bulges by one column in two phases:
-//   phase L: every bulge computes its left reflector (one lane each) and the workgroup applies
-//            all of them (their row sets are disjoint);
-//   phase R: every bulge computes its opposite (right) reflector from the updated 3x3 block of
-//            T and the workgroup applies all of them (disjoint column sets).
+// Every chase step advances ALL active bulges by one column:
+//   one lane per bulge computes its left reflector and, from the 3x3 block of T that this
+//   reflector produces (formed privately), its opposite (right) reflector;
+//   the workgroup applies all left reflectors (disjoint row sets), then all right reflectors
+//   (disjoint column sets), one wave per (bulge, matrix) segment.
 // Left and right multiplications commute, and no reflector reads an entry that a neighbouring
 // bulge's same-step reflector writes before it in the reference order (bottom bulge first,
 // PDHGEQZB.f:357-407; introduction order of PDHGEQZA.f:290-330), so the result equals the
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     double *bv = sV + (size_t)ld * w;          // per bulge: v1 v2 tau w1 w2 tau2 c s  (8)
     int *bc = (int *)(bv + 8 * CH_MAXB);       // per bulge: current tail column
     int *bk = bc + CH_MAXB;                    // per bulge: kind of this step (0 idle, 1 refl, 2 pushoff, 3 intro)
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *gH = a.H + GQZ_IX(a.p, a.p, a.ld);
     const double *gT = a.T + GQZ_IX(a.p, a.p, a.ld);
     for (int idx = tid; idx < w * w; idx += blockDim.x) {
@@ -108,100 +108,91 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
                 sH[(l - 1) + ld * (l - 2)] = r; sH[l + ld * (l - 2)] = 0.0;
                 B[6] = cs; B[7] = sn;
             }
+            if (kind == 1 || kind == 3) {
+                // the right reflector, from the 3x3 block of T after this bulge's left reflector
+                // (the other bulges' left reflectors act on other rows)
+                const int k = c + 1;
+                const double v1 = B[0], v2 = B[1], tau = B[2];
+                double r1[3], r2[3];
+                for (int jj = 0; jj < 3; ++jj) {
+                    const double *tc = sT + k + ld * (k + jj);
+                    const double t0 = tc[0], t1 = tc[1], t2 = tc[2];
+                    const double sj = tau * (t0 + v1 * t1 + v2 * t2);
+                    r1[jj] = t1 - sj * v1; r2[jj] = t2 - sj * v2;
+                }
+                double wv[3], tau2;
+                d_nullrefl3(r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], wv, tau2);
+                B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
+            } else if (kind == 2) {
+                // the closing column rotation, from T(l-1:l, l-1:l) after the row rotation
+                const int l = w - 1;
+                const double cs = B[6], sn = B[7];
+                const double x0 = sT[(l - 1) + ld * (l - 1)], y0 = sT[l + ld * (l - 1)];
+                const double x1 = sT[(l - 1) + ld * l], y1 = sT[l + ld * l];
+                double c2, s2, r;
+                d_lartgf(cs * y1 - sn * x1, cs * y0 - sn * x0, c2, s2, r);
+                B[3] = c2; B[4] = s2; B[5] = r;
+            }
             bc[b] = c; bk[b] = kind;
         }
         __syncthreads();
-        // apply all left transforms: per bulge 3w items (H cols, T cols, U rows)
-        for (int idx = tid, b = tid / (3 * w), r = tid - (tid / (3 * w)) * (3 * w); idx < nb * 3 * w;
-             idx += blockDim.x, r += blockDim.x) {
-            while (r >= 3 * w) { r -= 3 * w; ++b; }     // (b, r) = (idx / 3w, idx % 3w) without dividing
-            const int kind = bk[b];
-            if (kind == 0) continue;
-            const int c = bc[b];
+        // apply all left transforms: one wave per (bulge, matrix) segment; the lane is the column
+        // of H or T (rows k..k+2) or the row of U (columns k..k+2)
+        for (int seg = wave; seg < 3 * nb; seg += CH_NT / 64) {
+            const int b = seg / 3, typ = seg - 3 * b;
             const double *B = bv + 8 * b;
+            const int kind = bk[b], k = bc[b] + 1;
+            const double v1 = B[0], v2 = B[1], tau = B[2], cs = B[6], sn = B[7];
+            if (kind == 0 || lane >= w) continue;
             if (kind == 2) {
                 const int i = w - 2;
-                const double cs = B[6], sn = B[7];
-                if (r < w) {                         // H cols [w-2, w)
-                    const int j = r;
-                    if (j < w - 2) continue;
-                    double x = sH[i + ld * j], y = sH[(i + 1) + ld * j];
-                    sH[i + ld * j] = cs * x + sn * y; sH[(i + 1) + ld * j] = cs * y - sn * x;
-                } else if (r < 2 * w) {
-                    const int j = r - w;
-                    if (j < w - 2) continue;
-                    double x = sT[i + ld * j], y = sT[(i + 1) + ld * j];
-                    sT[i + ld * j] = cs * x + sn * y; sT[(i + 1) + ld * j] = cs * y - sn * x;
+                if (typ < 2) {
+                    if (lane < w - 2) continue;
+                    double *M = typ == 0 ? sH : sT;
+                    double x = M[i + ld * lane], y = M[(i + 1) + ld * lane];
+                    M[i + ld * lane] = cs * x + sn * y; M[(i + 1) + ld * lane] = cs * y - sn * x;
                 } else {
-                    const int row = r - 2 * w;
-                    double x = sU[row + ld * i], y = sU[row + ld * (i + 1)];
-                    sU[row + ld * i] = cs * x + sn * y; sU[row + ld * (i + 1)] = cs * y - sn * x;
+                    double x = sU[lane + ld * i], y = sU[lane + ld * (i + 1)];
+                    sU[lane + ld * i] = cs * x + sn * y; sU[lane + ld * (i + 1)] = cs * y - sn * x;
                 }
                 continue;
             }
-            const int k = c + 1;
-            const double v1 = B[0], v2 = B[1], tau = B[2];
             if (tau == 0.0) continue;
-            if (r < 2 * w) {
-                double *M = r < w ? sH : sT;
-                const int j = r < w ? r : r - w;
-                if (j < k) continue;                // columns k.. (column c set explicitly)
-                double *p = M + k + ld * j;
+            if (typ < 2) {
+                if (lane < k) continue;                 // columns k.. (column c set explicitly)
+                double *p = (typ == 0 ? sH : sT) + k + ld * lane;
                 double t = tau * (p[0] + v1 * p[1] + v2 * p[2]);
                 p[0] -= t; p[1] -= t * v1; p[2] -= t * v2;
             } else {
-                const int row = r - 2 * w;
-                double *x = sU + row + ld * k;
+                double *x = sU + lane + ld * k;
                 double x0 = x[0], x1 = x[ld], x2 = x[2 * ld];
                 double t = tau * (x0 + v1 * x1 + v2 * x2);
                 x[0] = x0 - t; x[ld] = x1 - t * v1; x[2 * ld] = x2 - t * v2;
             }
         }
         __syncthreads();
-        // ---------------- phase R: right reflectors ----------------
-        if (tid < nb) {
-            const int b = tid, kind = bk[b];
-            double *B = bv + 8 * b;
-            if (kind == 1 || kind == 3) {
-                const int k = bc[b] + 1;
-                double wv[3], tau2;
-                const double *t = sT + k + ld * k;
-                d_nullrefl3(t[1], t[1 + ld], t[1 + 2 * ld], t[2], t[2 + ld], t[2 + 2 * ld], wv, tau2);
-                B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
-            } else if (kind == 2) {
-                const int l = w - 1;
-                double cs, sn, r;
-                d_lartgf(sT[l + ld * l], sT[l + ld * (l - 1)], cs, sn, r);
-                sT[l + ld * l] = r; sT[l + ld * (l - 1)] = 0.0;
-                B[6] = cs; B[7] = sn;
-            }
-        }
-        __syncthreads();
-        for (int idx = tid, b = tid / (3 * w), r = tid - (tid / (3 * w)) * (3 * w); idx < nb * 3 * w;
-             idx += blockDim.x, r += blockDim.x) {
-            while (r >= 3 * w) { r -= 3 * w; ++b; }
-            const int kind = bk[b];
-            if (kind == 0) continue;
+        // apply all right transforms: one wave per (bulge, matrix) segment; the lane is the row
+        for (int seg = wave; seg < 3 * nb; seg += CH_NT / 64) {
+            const int b = seg / 3, typ = seg - 3 * b;
             const double *B = bv + 8 * b;
+            const int kind = bk[b], k = bc[b] + 1;
+            const double w1 = B[3], w2 = B[4], tau2 = B[5];    // kind 2: c, s, r of the rotation
+            if (kind == 0 || lane >= w) continue;
+            double *M = typ == 0 ? sH : (typ == 1 ? sT : sV);
             if (kind == 2) {
                 const int l = w - 1;
-                const double cs = B[6], sn = B[7];
-                double *M; int row;
-                if (r < w) { M = sH; row = r; }
-                else if (r < 2 * w) { M = sT; row = r - w; if (row >= l) continue; }
-                else { M = sV; row = r - 2 * w; }
-                double x = M[row + ld * l], y = M[row + ld * (l - 1)];
-                M[row + ld * l] = cs * x + sn * y; M[row + ld * (l - 1)] = cs * y - sn * x;
+                if (typ == 1 && lane >= l) {
+                    if (lane == l) { sT[l + ld * l] = tau2; sT[l + ld * (l - 1)] = 0.0; }
+                    continue;
+                }
+                double x = M[lane + ld * l], y = M[lane + ld * (l - 1)];
+                M[lane + ld * l] = w1 * x + w2 * y; M[lane + ld * (l - 1)] = w1 * y - w2 * x;
                 continue;
             }
-            const int k = bc[b] + 1;
-            const double w1 = B[3], w2 = B[4], tau2 = B[5];
             if (tau2 == 0.0) continue;
-            double *M; int row;
-            if (r < w) { M = sH; row = r; if (row >= min(k + 4, w)) continue; }
-            else if (r < 2 * w) { M = sT; row = r - w; if (row >= k + 3) continue; }
-            else { M = sV; row = r - 2 * w; }
-            double *x = M + row + ld * k;
+            if (typ == 0 && lane >= min(k + 4, w)) continue;
+            if (typ == 1 && lane >= k + 3) continue;
+            double *x = M + lane + ld * k;
             double x0 = x[0], x1 = x[ld], x2 = x[2 * ld];
             double t = tau2 * (x0 + w1 * x1 + w2 * x2);
             x[0] = x0 - t; x[ld] = x1 - t * w1; x[2 * ld] = x2 - t * w2;
