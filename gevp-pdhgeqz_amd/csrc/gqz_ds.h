@@ -156,11 +156,12 @@ static __device__ void wg_ds_sweep(const WgPencil &P, WgCtl *C, int f, int l, do
     __syncthreads();
     wg_colrot_all(P, l, l - 1, l + 1, l, C->c, C->s);
     __syncthreads();
-    if (tid == 0) {
-        // vigilant deflation (DHGEQZ5.f:254-275)
+    {
+        // vigilant deflation (DHGEQZ5.f:254-275); the test of H(j, j-1) reads no entry that
+        // another test writes, so the workgroup runs them all at once
         double *H = P.H;
         int ld = P.ld;
-        for (int j = f + 1; j <= l; ++j) {
+        for (int j = f + 1 + tid; j <= l; j += blockDim.x) {
             double hj = H[GQZ_IX(j, j - 1, ld)];
             if (hj == 0.0) continue;
             double a = H[GQZ_IX(j - 1, j - 1, ld)], b = H[GQZ_IX(j, j, ld)];

@@ -42,12 +42,26 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *gH = a.H + GQZ_IX(a.p, a.p, a.ld);
     const double *gT = a.T + GQZ_IX(a.p, a.p, a.ld);
-    for (int idx = tid; idx < w * w; idx += blockDim.x) {
-        int i = idx % w, j = idx / w;
-        sH[i + ld * j] = gH[GQZ_IX(i, j, a.ld)];
-        sT[i + ld * j] = gT[GQZ_IX(i, j, a.ld)];
-        sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
-        sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+    // window load: every load of a thread is issued before its first LDS write
+    {
+        double hv[CH_MAXW * CH_MAXW / CH_NT], tv[CH_MAXW * CH_MAXW / CH_NT];
+#pragma unroll
+        for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
+            const bool ok = idx < w * w;
+            hv[q] = ok ? gH[GQZ_IX(i, j, a.ld)] : 0.0;
+            tv[q] = ok ? gT[GQZ_IX(i, j, a.ld)] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
+            if (idx < w * w) {
+                sH[i + ld * j] = hv[q];
+                sT[i + ld * j] = tv[q];
+                sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
+                sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+            }
+        }
     }
     __syncthreads();
     const int total = a.intro ? 3 * nb - 2 + a.steps : a.steps;
@@ -211,8 +225,9 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
         if (k + 2 <= w - 1 && k >= 0) { sT[(k + 1) + ld * k] = 0.0; sT[(k + 2) + ld * k] = 0.0; }
     }
     __syncthreads();
-    if (tid == 0) {
-        for (int j = 1; j < w; ++j) {
+    {
+        // (the test of H(j, j-1) reads no entry that another test writes: all at once)
+        for (int j = 1 + tid; j < w; j += blockDim.x) {
             double hj = sH[j + ld * (j - 1)];
             if (hj == 0.0) continue;
             double x = sH[(j - 1) + ld * (j - 1)], y = sH[j + ld * j];

@@ -139,6 +139,150 @@ __global__ void __launch_bounds__(256) k_slab_left(GqzSlabJobs J, int kmax) {
     }
 }
 
+// ---- factors of order k <= 64: streaming tiles through MFMA registers --------------------------
+// A workgroup stages the factor in LDS once, in MFMA operand order (entry [t][s][lane] is what
+// lane `lane` feeds to the k-step s of output tile t: conflict-free 512-byte reads); each of its
+// four waves then streams 16-row (right update) or 16-column (left update) tiles of the slab:
+// the tile is read once into registers, multiplied (at most 64 MFMAs) and written back, with no
+// further barrier.  The products are formed transposed (D^T = F^T C^T for the right update,
+// D^T = C^T U for the left one), so that the D layout stores 16 consecutive rows of a column:
+// full 128-byte lines.
+#define SLAB_RK 64
+
+// (all 16 loads of a lane are issued before the first LDS write: one memory latency, not 16)
+static __device__ __forceinline__ void stage_factor(double *sF, const double *__restrict__ F, int ldf, int k) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q, r = idx & 63, c = idx >> 6;   // F[r][c], coalesced along r
+        v[q] = (r < k && c < k) ? F[r + (long)c * ldf] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q, r = idx & 63, c = idx >> 6;
+        const int t = c >> 4, li = c & 15, s = r >> 2, lk = r & 3;
+        sF[((t * 16 + s) << 6) + lk * 16 + li] = v[q];
+    }
+    __syncthreads();
+}
+
+// C[m x k] <- C F : tile = 16 rows of C; D^T[j][i] = sum_kk F[kk][j] C[i][kk].
+// NJ = ceil(kmax / 16) output tiles of 16 columns, 4 NJ k-steps; the operands are zero beyond k,
+// so the MFMA sequence is straight-line code (no guard between the MFMAs of a tile).
+template <int NJ>
+__global__ void __launch_bounds__(256) k_slab_right_reg(GqzSlabJobs J, int tpw) {
+    __shared__ double sF[64 * 64];
+    int bx = blockIdx.x;
+    const int jb = slab_job(J, bx);
+    double *__restrict__ C = J.C[jb];
+    const int ldc = J.ldc[jb], m = J.m[jb], k = J.k[jb];
+    stage_factor(sF, J.F[jb], J.ldf[jb], k);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long tb = ((long)bx * 4 + wave) * tpw;        // first tile of this wave
+    for (int t = 0; t < tpw; ++t) {
+        const long r0 = (tb + t) * 16;
+        if (r0 >= m) break;
+        const long row = r0 + li;
+        const bool rok = row < m;
+        double b[4 * NJ];
+#pragma unroll
+        for (int s = 0; s < 4 * NJ; ++s) {
+            const int c = 4 * s + lk;
+            b[s] = (rok && c < k) ? C[row + (long)c * ldc] : 0.0;
+        }
+        dbl4 acc[NJ];
+#pragma unroll
+        for (int jt = 0; jt < NJ; ++jt) acc[jt] = dbl4{ 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+        for (int s = 0; s < 4 * NJ; ++s)
+#pragma unroll
+            for (int jt = 0; jt < NJ; ++jt) acc[jt] = mfma16(sF[((jt * 16 + s) << 6) + lane], b[s], acc[jt]);
+        if (rok) {
+#pragma unroll
+            for (int jt = 0; jt < NJ; ++jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = 16 * jt + lk + 4 * r;
+                    if (c < k) C[row + (long)c * ldc] = acc[jt][r];
+                }
+        }
+    }
+}
+
+// C[k x n] <- U^T C : tile = 16 columns of C; D^T[j][i] = sum_kk C[kk][j] U[kk][i]
+template <int NJ>
+__global__ void __launch_bounds__(256) k_slab_left_reg(GqzSlabJobs J, int tpw) {
+    __shared__ double sU[64 * 64];
+    int bx = blockIdx.x;
+    const int jb = slab_job(J, bx);
+    double *__restrict__ C = J.C[jb];
+    const int ldc = J.ldc[jb], n = J.m[jb], k = J.k[jb];
+    stage_factor(sU, J.F[jb], J.ldf[jb], k);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long tb = ((long)bx * 4 + wave) * tpw;
+    for (int t = 0; t < tpw; ++t) {
+        const long c0 = (tb + t) * 16;
+        if (c0 >= n) break;
+        const long col = c0 + li;                      // column of this lane's A operand
+        const bool cok = col < n;
+        double a[4 * NJ];
+#pragma unroll
+        for (int s = 0; s < 4 * NJ; ++s) {
+            const int r = 4 * s + lk;
+            a[s] = (cok && r < k) ? C[r + col * (long)ldc] : 0.0;
+        }
+        dbl4 acc[NJ];
+#pragma unroll
+        for (int it = 0; it < NJ; ++it) acc[it] = dbl4{ 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+        for (int s = 0; s < 4 * NJ; ++s)
+#pragma unroll
+            for (int it = 0; it < NJ; ++it) acc[it] = mfma16(a[s], sU[((it * 16 + s) << 6) + lane], acc[it]);
+        // D^T lane l, element r: column lk + 4r of the tile, row 16 it + li
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long cj = c0 + lk + 4 * r;
+            if (cj >= n) continue;
+#pragma unroll
+            for (int it = 0; it < NJ; ++it) {
+                const int i = 16 * it + li;
+                if (i < k) C[i + cj * (long)ldc] = acc[it][r];
+            }
+        }
+    }
+}
+
+static int slab_kmax(const GqzSlabJobs &J);
+static void launch_reg(hipStream_t st, GqzSlabJobs &J, bool right) {
+    long tiles = 0;
+    for (int q = 0; q < J.njobs; ++q) tiles += (J.m[q] + 15) / 16;
+    // tiles per wave: about 2 workgroups per CU before runs get longer, at most 8 tiles per wave
+    const int tpw = (int)min(8L, max(1L, tiles / 2048));
+    int tot = 0;
+    for (int q = 0; q < J.njobs; ++q) {
+        J.blk0[q] = tot;
+        const int t = (J.m[q] + 15) / 16;
+        tot += (t + 4 * tpw - 1) / (4 * tpw);
+    }
+    J.blk0[J.njobs] = tot;
+    if (tot == 0) return;
+    const int nj = (slab_kmax(J) + 15) / 16;
+#define GQZ_SLAB_REG(NJ)                                                                             \
+    do {                                                                                             \
+        if (right) hipLaunchKernelGGL(k_slab_right_reg<NJ>, dim3(tot), dim3(256), 0, st, J, tpw);    \
+        else hipLaunchKernelGGL(k_slab_left_reg<NJ>, dim3(tot), dim3(256), 0, st, J, tpw);           \
+    } while (0)
+    switch (nj) {
+    case 1: GQZ_SLAB_REG(1); break;
+    case 2: GQZ_SLAB_REG(2); break;
+    case 3: GQZ_SLAB_REG(3); break;
+    default: GQZ_SLAB_REG(4); break;
+    }
+#undef GQZ_SLAB_REG
+}
+
 bool gqz_slab_add(GqzSlabJobs &J, double *C, int ldc, int m, const double *F, int ldf, int k) {
     if (m <= 0 || k <= 0) return true;
     if (J.njobs >= GQZ_SLAB_MAXJOBS) return false;
@@ -185,14 +329,16 @@ static void launch_left(hipStream_t st, GqzSlabJobs &J, int kmax) {
 void gqz_slab_right_multi(hipStream_t st, GqzSlabJobs &J) {
     if (J.njobs <= 0) return;
     const int k = slab_kmax(J);
-    if (k <= 128) launch_right<64>(st, J, k);
+    if (k <= SLAB_RK) launch_reg(st, J, true);
+    else if (k <= 128) launch_right<64>(st, J, k);
     else if (k <= 256) launch_right<32>(st, J, k);
     else launch_right<16>(st, J, k);
 }
 void gqz_slab_left_multi(hipStream_t st, GqzSlabJobs &J) {
     if (J.njobs <= 0) return;
     const int k = slab_kmax(J);
-    if (k <= 128) launch_left<64>(st, J, k);
+    if (k <= SLAB_RK) launch_reg(st, J, false);
+    else if (k <= 128) launch_left<64>(st, J, k);
     else if (k <= 256) launch_left<32>(st, J, k);
     else launch_left<16>(st, J, k);
 }
