@@ -377,18 +377,20 @@ static __device__ int wg_qz_ds(const WgPencil &P, WgCtl *C, int ilo, int ihi, do
 // --------------------------------------------------------------------------------------------
 static __device__ void wg_gghrd(double *A, double *B, int ld, int n, int ilo, int ihi, double *Q, double *Z,
                                 int ldq, int nq, double *sh) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     for (int j = ilo; j <= ihi - 2; ++j) {
         for (int i = ihi; i >= j + 2; --i) {
             if (tid == 0) {
+                // all reads first (none of them is written before it is read), then the arithmetic
+                const double f = A[GQZ_IX(i - 1, j, ld)], g = A[GQZ_IX(i, j, ld)];
+                const double a00 = A[GQZ_IX(i - 1, i - 1, ld)], a01 = A[GQZ_IX(i - 1, i, ld)];
+                const double a10 = A[GQZ_IX(i, i - 1, ld)], a11 = A[GQZ_IX(i, i, ld)];
+                const double b00 = B[GQZ_IX(i - 1, i - 1, ld)], b01 = B[GQZ_IX(i - 1, i, ld)];
+                const double b11 = B[GQZ_IX(i, i, ld)];
                 double c, s, r, cc, ss;
-                d_lartgf(A[GQZ_IX(i - 1, j, ld)], A[GQZ_IX(i, j, ld)], c, s, r);
+                d_lartgf(f, g, c, s, r);
                 A[GQZ_IX(i - 1, j, ld)] = r; A[GQZ_IX(i, j, ld)] = 0.0;
                 // 2x2 corners (rows i-1,i x cols i-1,i) of A and B: row rotation first
-                double a00 = A[GQZ_IX(i - 1, i - 1, ld)], a01 = A[GQZ_IX(i - 1, i, ld)];
-                double a10 = A[GQZ_IX(i, i - 1, ld)], a11 = A[GQZ_IX(i, i, ld)];
-                double b00 = B[GQZ_IX(i - 1, i - 1, ld)], b01 = B[GQZ_IX(i - 1, i, ld)];
-                double b11 = B[GQZ_IX(i, i, ld)];
                 double na00 = c * a00 + s * a10, na10 = c * a10 - s * a00;
                 double na01 = c * a01 + s * a11, na11 = c * a11 - s * a01;
                 double nb00 = c * b00, nb10 = -s * b00;
@@ -407,49 +409,57 @@ static __device__ void wg_gghrd(double *A, double *B, int ld, int n, int ilo, in
             const double c = sh[0], s = sh[1], cc = sh[2], ss = sh[3];
             // row rotation: A cols [j+1, n) \ {i-1, i}; B cols [i+1, n); Q cols (i-1, i)
             // column rotation: A rows [0, ihi] \ {i-1, i}; B rows [0, i-1); Z cols (i, i-1)
+            // Every item is a pair of entries (x, y) -> (c x + s y, c y - s x); a thread gathers
+            // four items and issues all their loads before its first store.
             const int na = n - (j + 1), nb = n - (i + 1), nc = ihi + 1, nd = i - 1;
             const int nqq = Q ? nq : 0, nzz = Z ? nq : 0;
             const int tot = na + nb + nqq + nc + nd + nzz;
-            for (int idx = tid; idx < tot; idx += blockDim.x) {
-                int t = idx;
-                if (t < na) {
-                    int col = j + 1 + t;
-                    if (col == i - 1 || col == i) continue;
-                    double x = A[GQZ_IX(i - 1, col, ld)], y = A[GQZ_IX(i, col, ld)];
-                    A[GQZ_IX(i - 1, col, ld)] = c * x + s * y; A[GQZ_IX(i, col, ld)] = c * y - s * x;
-                    continue;
+            for (int base = tid; base < tot; base += 4 * nt) {
+                double *px[4], *py[4], cq[4], sq[4];
+                bool ok[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int t = base + q * nt;
+                    ok[q] = t < tot; px[q] = py[q] = A; cq[q] = c; sq[q] = s;
+                    if (!ok[q]) continue;
+                    if (t < na) {
+                        const int col = j + 1 + t;
+                        ok[q] = (col != i - 1 && col != i);
+                        px[q] = A + GQZ_IX(i - 1, col, ld); py[q] = A + GQZ_IX(i, col, ld);
+                        continue;
+                    }
+                    t -= na;
+                    if (t < nb) {
+                        const int col = i + 1 + t;
+                        px[q] = B + GQZ_IX(i - 1, col, ld); py[q] = B + GQZ_IX(i, col, ld);
+                        continue;
+                    }
+                    t -= nb;
+                    if (t < nqq) { px[q] = Q + GQZ_IX(t, i - 1, ldq); py[q] = Q + GQZ_IX(t, i, ldq); continue; }
+                    t -= nqq;
+                    cq[q] = cc; sq[q] = ss;
+                    if (t < nc) {
+                        ok[q] = (t != i - 1 && t != i);
+                        px[q] = A + GQZ_IX(t, i, ld); py[q] = A + GQZ_IX(t, i - 1, ld);
+                        continue;
+                    }
+                    t -= nc;
+                    if (t < nd) { px[q] = B + GQZ_IX(t, i, ld); py[q] = B + GQZ_IX(t, i - 1, ld); continue; }
+                    t -= nd;
+                    px[q] = Z + GQZ_IX(t, i, ldq); py[q] = Z + GQZ_IX(t, i - 1, ldq);
                 }
-                t -= na;
-                if (t < nb) {
-                    int col = i + 1 + t;
-                    double x = B[GQZ_IX(i - 1, col, ld)], y = B[GQZ_IX(i, col, ld)];
-                    B[GQZ_IX(i - 1, col, ld)] = c * x + s * y; B[GQZ_IX(i, col, ld)] = c * y - s * x;
-                    continue;
+                double x[4], y[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    x[q] = ok[q] ? *px[q] : 0.0;
+                    y[q] = ok[q] ? *py[q] : 0.0;
                 }
-                t -= nb;
-                if (t < nqq) {
-                    double x = Q[GQZ_IX(t, i - 1, ldq)], y = Q[GQZ_IX(t, i, ldq)];
-                    Q[GQZ_IX(t, i - 1, ldq)] = c * x + s * y; Q[GQZ_IX(t, i, ldq)] = c * y - s * x;
-                    continue;
-                }
-                t -= nqq;
-                if (t < nc) {
-                    if (t == i - 1 || t == i) continue;
-                    double x = A[GQZ_IX(t, i, ld)], y = A[GQZ_IX(t, i - 1, ld)];
-                    A[GQZ_IX(t, i, ld)] = cc * x + ss * y; A[GQZ_IX(t, i - 1, ld)] = cc * y - ss * x;
-                    continue;
-                }
-                t -= nc;
-                if (t < nd) {
-                    double x = B[GQZ_IX(t, i, ld)], y = B[GQZ_IX(t, i - 1, ld)];
-                    B[GQZ_IX(t, i, ld)] = cc * x + ss * y; B[GQZ_IX(t, i - 1, ld)] = cc * y - ss * x;
-                    continue;
-                }
-                t -= nd;
-                {
-                    double x = Z[GQZ_IX(t, i, ldq)], y = Z[GQZ_IX(t, i - 1, ldq)];
-                    Z[GQZ_IX(t, i, ldq)] = cc * x + ss * y; Z[GQZ_IX(t, i - 1, ldq)] = cc * y - ss * x;
-                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (ok[q]) {
+                        *px[q] = cq[q] * x[q] + sq[q] * y[q];
+                        *py[q] = cq[q] * y[q] - sq[q] * x[q];
+                    }
             }
             __syncthreads();
         }
