@@ -9,9 +9,11 @@ values.  The QZ arithmetic is unchanged: only the three counts move, every resul
 the backward-error check.
 
 Training data come from the QZ itself: for each training pencil a small grid of static multipliers
-is run, every top-level iteration is recorded (gqz_trace_get), and the multipliers of the fastest run
-become the targets for the features seen in the fastest and in the default run.  Training uses the
-library's own FP32 gradient and Adam kernels (gqz_mlp_train_step_dev) on the GPU.
+is run and every top-level iteration is recorded (gqz_trace_get).  The fastest run fixes the target:
+its first AED window W*; every recorded state of the pencil gets the NW target log2(W* / NW_def(n))
+(the defaults are evaluated once per QZ call, at the order n of the active block) and NS, NIBBLE
+targets 0.  Training uses the library's own FP32 gradient and Adam kernels
+(gqz_mlp_train_step_dev) on the GPU.
 """
 import ctypes as C
 import math
@@ -27,6 +29,22 @@ MODEL = os.path.join(HERE, "mlp_policy.npz")
 NPAR = 5187
 FEATURES = ["log2(nh)/16", "nh/n", "nd/jw last", "nd/jw previous", "nd/jw mean of 3", "inf. deflations/nh",
             "NS/nh last", "NW/nh last", "NIBBLE/100 last", "swept last", "iterations/200", "log2(n)/16"]
+
+
+# ---- PIPARMQ defaults (piparmq.f:445-476, as in qz_host.cpp) -------------------------------------------
+def piparmq_ns(nh):
+    ns = 2
+    if nh >= 30: ns = 4
+    if nh >= 60: ns = 10
+    if nh >= 150: ns = max(10, nh // int(round(math.log(nh) / math.log(2.0))))
+    for lim, v in ((590, 64), (3000, 128), (6000, 256), (12000, 512), (24000, 1024), (48000, 2048)):
+        if nh >= lim: ns = v
+    return max(2, ns - ns % 2)
+
+
+def piparmq_nw(nh):
+    ns = piparmq_ns(nh)
+    return ns if nh <= 500 else 3 * ns // 2
 
 
 # ---- parameters --------------------------------------------------------------------------------------

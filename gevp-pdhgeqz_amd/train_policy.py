@@ -1,11 +1,13 @@
 """Generate QZ traces on the GPU and train the learned parameter policy (writes mlp_policy.npz).
 
-    python gevp-pdhgeqz_amd/train_policy.py [--sizes 1500,2500,4000] [--families ht,graded,singular]
+    python gevp-pdhgeqz_amd/train_policy.py [--sizes 2000,4000,8000] [--families ht,graded,singular]
                                             [--seeds 1] [--steps 3000] [--out PATH]
 
 For every training pencil the QZ runs once per multiplier triple of the grid (NS x NW x NIBBLE
-relative to the PIPARMQ defaults); the fastest run's log2 multipliers become the regression targets
-of the features recorded in the fastest and in the default run.  The perceptron is then fitted with
+relative to the PIPARMQ defaults, which the QZ evaluates once per call at the order of the active
+block).  The first AED window W* of the fastest run is the target: every state recorded in the runs
+of that pencil is labelled log2(W* / NW_def(n)) for NW and 0 for NS and NIBBLE.  Larger pencils (--large-sizes, one family, a reduced grid) extend the
+range of active orders seen in training.  The perceptron is then fitted with
 the library's own FP32 gradient and Adam kernels.  Each run's backward error is checked: a run that
 fails it is never a target.
 """
@@ -25,9 +27,12 @@ if HERE not in sys.path:
 import gqz  # noqa: E402
 import gqz_policy as GP  # noqa: E402
 
-GRID_NS = (0.5, 1.0, 2.0)
-GRID_NW = (0.25, 0.5, 1.0)
+# NS beyond the shifts an AED window delivers has no effect, so the grid spans the window size: from
+# windows that fit the LDS-resident AED kernel to windows with a nested QZ
+GRID_NS = (1.0,)
+GRID_NW = (0.03125, 0.0625, 0.125, 0.25)
 GRID_NIB = (1.0,)
+GRID_NW_LARGE = (0.0625, 0.125)
 
 
 def log(*a):
@@ -36,9 +41,11 @@ def log(*a):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sizes", default="1500,2500,4000")
+    ap.add_argument("--sizes", default="2000,4000,8000")
     ap.add_argument("--families", default="ht,graded,singular")
     ap.add_argument("--seeds", default="1")
+    ap.add_argument("--large-sizes", default="12000,16000")
+    ap.add_argument("--large-families", default="ht")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=GP.MODEL)
     args = ap.parse_args()
@@ -49,10 +56,13 @@ def main():
     fams = args.families.split(",")
     seeds = [int(x) for x in args.seeds.split(",")]
     X, Y, summary = [], [], []
-    for fam, n, seed in itertools.product(fams, sizes, seeds):
+    cases = [(fam, n, seed, GRID_NW) for fam, n, seed in itertools.product(fams, sizes, seeds)]
+    large = [int(x) for x in args.large_sizes.split(",") if x]
+    cases += [(fam, n, seeds[0], GRID_NW_LARGE) for fam in args.large_families.split(",") for n in large]
+    for fam, n, seed, grid_nw in cases:
         H0, T0 = GP.make_pencil(fam, n, 1000 * seed + n)
         runs = []
-        for mult in itertools.product(GRID_NS, GRID_NW, GRID_NIB):
+        for mult in itertools.product(GRID_NS, grid_nw, GRID_NIB):
             dt, info, rows = GP.run_traced(L, H0, T0, mult=mult)
             runs.append((dt, info, mult, rows))
             log("trace %s n=%d seed=%d mult=%s: %.2f s info=%d iterations=%d" % (fam, n, seed, mult, dt, info,
@@ -60,10 +70,11 @@ def main():
         ok = [r for r in runs if r[1] == 0]
         best = min(ok, key=lambda r: r[0])
         dflt = [r for r in ok if r[2] == (1.0, 1.0, 1.0)]
-        target = np.log2(np.array(best[2], dtype=np.float32))
-        for r in [best] + dflt:
+        wstar = best[3][0][3] if best[3] else GP.piparmq_nw(n)     # first AED window of the fastest run
+        for r in ok:                     # every recorded state of every run of this pencil
             for row in r[3]:
-                X.append(row[0]); Y.append(target)
+                t_nw = float(np.clip(np.log2(wstar / GP.piparmq_nw(n)), -5.0, 1.0))
+                X.append(row[0]); Y.append(np.array([0.0, t_nw, 0.0], dtype=np.float32))
         summary.append(dict(family=fam, n=n, seed=seed, best=list(best[2]), best_s=best[0],
                             default_s=dflt[0][0] if dflt else None))
         log("best %s n=%d: mult=%s %.2f s (default %.2f s)" % (fam, n, best[2], best[0],
