@@ -30,6 +30,32 @@ for p in (os.path.join(ROOT, "gevp-pdhgeqz_amd"), os.path.join(ROOT, "tests"), R
 PEAK_HBM = 8.0e12          # B/s, MI355X HBM3E
 PEAK_FP64 = 78.6e12        # flop/s, MI355X FP64 matrix (dense)
 RIDGE = PEAK_FP64 / PEAK_HBM
+# hardware byte counts per dispatch of the QZ kernels on the default workload (two rocprofv3 --pmc
+# passes, FETCH_SIZE and WRITE_SIZE, scaled by the same counters on kernels of known size)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "qz_n16000_learned_traffic.json")
+
+
+def class_traffic(name, n):
+    """HBM bytes per launch of a kernel class from the committed counter summary (None when the
+    summary does not cover this workload or class)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("n") != n or t.get("params") is None:
+        return None
+    kern = t["classes"].get(name.split("@")[0])
+    if not kern:
+        return None
+    ff, wf = t["calibration"]["fetch_factor"], t["calibration"]["write_factor"]
+    tot, cnt = 0.0, 0
+    for k in kern:
+        e = t["kernels"].get(k)
+        if e:
+            tot += e["dispatches"] * (e["fetch_kib"] * ff + e["write_kib"] * wf) * 1024.0
+            cnt += e["dispatches"]
+    return tot / cnt if cnt else None
 
 
 class KTime(C.Structure):
@@ -236,7 +262,7 @@ def main():
             ach = c["bytes"] / c["launches"] / avg_s / 1e9
             r = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
         r["frac"] = r["achieved"] / r["peak"]
-        r["traffic"] = None
+        r["traffic"] = class_traffic(c["name"], n) if args.params == "learned" else None
         r["kernel"] = c["name"]
         r["avg_launch_us"] = 1e6 * avg_s
         r["launches_per_step"] = c["launches"] / args.steps
