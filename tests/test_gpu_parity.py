@@ -34,7 +34,8 @@ def _note(*a):
 
 
 # ---- (d) slab updates: FP64 MFMA in-place products -------------------------------------------------
-@pytest.mark.parametrize("m,k", [(16, 4), (300, 64), (257, 200), (130, 520), (5, 7), (1000, 130)])
+@pytest.mark.parametrize("m,k", [(16, 4), (300, 64), (257, 200), (130, 520), (5, 7), (1000, 130),
+                                 (33, 24), (1000, 48), (70001, 61), (70000, 64)])
 def test_slab_update_matches_fp64_product(m, k):
     torch, gqz, _ = _mods()
     _note("slab", m, k)
