@@ -82,70 +82,73 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
             // shifts of this bulge (pair index nb-1-b)
             const int sp = 2 * (nb - 1 - b);
             double *B = bv + 8 * b;
-            if (kind == 3) {
-                double v[3];
-                d_shift_col(sH, ld, sT, ld, a.sr[sp], a.si[sp], a.sb[sp], a.sr[sp + 1], a.sb[sp + 1], v);
-                double al = v[0], x1 = v[1], x2 = v[2], tau;
-                d_larfg3f(al, x1, x2, tau);
-                B[0] = x1; B[1] = x2; B[2] = tau;
-            } else if (kind == 1) {
+            if (kind == 1 || kind == 3) {
                 const int k = c + 1;
-                // zero the fill left in T by this bulge's previous right reflector
-                if (q > 0) { sT[k + ld * (k - 1)] = 0.0; sT[(k + 1) + ld * (k - 1)] = 0.0; }
-                double h0 = sH[k + ld * c], h1 = sH[(k + 1) + ld * c], h2 = sH[(k + 2) + ld * c];
-                double beta = h0, x1 = h1, x2 = h2, tau;
-                d_larfg3f(beta, x1, x2, tau);
-                if (h2 != 0.0 || sH[(k + 2) + ld * k] != 0.0 || sH[(k + 2) + ld * (k + 1)] == 0.0) {
-                    sH[k + ld * c] = beta;
-                } else {
-                    double wv[3];
-                    d_shift_col(sH + k + ld * k, ld, sT + k + ld * k, ld, a.sr[sp], a.si[sp], a.sb[sp],
-                                a.sr[sp + 1], a.sb[sp + 1], wv);
-                    double a1 = wv[0], y1 = wv[1], y2 = wv[2], tau1;
-                    d_larfg3f(a1, y1, y2, tau1);
-                    double refsum = tau1 * (h0 + y1 * h1);
-                    double tst = fabs(sH[c + ld * c]) + fabs(sH[k + ld * k]) + fabs(sH[(k + 1) + ld * (k + 1)]);
-                    if (fabs(h1 - refsum * y1) + fabs(refsum * y2) > GQZ_ULP * tst) {
-                        sH[k + ld * c] = beta;
-                    } else {
-                        sH[k + ld * c] = h0 - refsum;
-                        tau = tau1; x1 = y1; x2 = y2;
-                    }
+                // all reads first (the writes below touch column c of H and column k-1 of T only),
+                // then the chain: left reflector, the 3x3 block of T after it (the other bulges'
+                // left reflectors act on other rows), right reflector
+                double t3[9];
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                    for (int ii = 0; ii < 3; ++ii) t3[ii + 3 * jj] = sT[(k + ii) + ld * (k + jj)];
+                double h0 = 0.0, h1 = 0.0, h2 = 0.0, hk2 = 0.0, hk21 = 0.0;
+                if (kind == 1) {
+                    h0 = sH[k + ld * c]; h1 = sH[(k + 1) + ld * c]; h2 = sH[(k + 2) + ld * c];
+                    hk2 = sH[(k + 2) + ld * k]; hk21 = sH[(k + 2) + ld * (k + 1)];
                 }
-                sH[(k + 1) + ld * c] = 0.0; sH[(k + 2) + ld * c] = 0.0;
-                B[0] = x1; B[1] = x2; B[2] = tau;
+                double x1, x2, tau;
+                if (kind == 3) {
+                    double v[3];
+                    d_shift_col(sH, ld, sT, ld, a.sr[sp], a.si[sp], a.sb[sp], a.sr[sp + 1], a.sb[sp + 1], v);
+                    double al = v[0];
+                    x1 = v[1]; x2 = v[2];
+                    d_larfg3f(al, x1, x2, tau);
+                } else {
+                    // zero the fill left in T by this bulge's previous right reflector
+                    if (q > 0) { sT[k + ld * (k - 1)] = 0.0; sT[(k + 1) + ld * (k - 1)] = 0.0; }
+                    double beta = h0;
+                    x1 = h1; x2 = h2;
+                    d_larfg3f(beta, x1, x2, tau);
+                    double hnew = beta;
+                    if (h2 == 0.0 && hk2 == 0.0 && hk21 != 0.0) {
+                        // collapsed bulge: try to re-introduce it (DHGEQZ5.f:141-189)
+                        double wv[3];
+                        d_shift_col(sH + k + ld * k, ld, sT + k + ld * k, ld, a.sr[sp], a.si[sp], a.sb[sp],
+                                    a.sr[sp + 1], a.sb[sp + 1], wv);
+                        double a1 = wv[0], y1 = wv[1], y2 = wv[2], tau1;
+                        d_larfg3f(a1, y1, y2, tau1);
+                        double refsum = tau1 * (h0 + y1 * h1);
+                        double tst = fabs(sH[c + ld * c]) + fabs(sH[k + ld * k]) + fabs(sH[(k + 1) + ld * (k + 1)]);
+                        if (!(fabs(h1 - refsum * y1) + fabs(refsum * y2) > GQZ_ULP * tst)) {
+                            hnew = h0 - refsum;
+                            tau = tau1; x1 = y1; x2 = y2;
+                        }
+                    }
+                    sH[k + ld * c] = hnew; sH[(k + 1) + ld * c] = 0.0; sH[(k + 2) + ld * c] = 0.0;
+                }
+                double r1[3], r2[3];
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) {
+                    const double t0 = t3[3 * jj], t1 = t3[1 + 3 * jj], t2 = t3[2 + 3 * jj];
+                    const double sj = tau * (t0 + x1 * t1 + x2 * t2);
+                    r1[jj] = t1 - sj * x1; r2[jj] = t2 - sj * x2;
+                }
+                double wv[3], tau2;
+                d_nullrefl3(r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], wv, tau2);
+                B[0] = x1; B[1] = x2; B[2] = tau; B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
             } else if (kind == 2) {
                 const int l = w - 1;
+                const double x0 = sT[(l - 1) + ld * (l - 1)], y0 = sT[l + ld * (l - 1)];
+                const double xx1 = sT[(l - 1) + ld * l], yy1 = sT[l + ld * l];
                 if (q > 0) { sT[(l - 1) + ld * (l - 2)] = 0.0; sT[l + ld * (l - 2)] = 0.0; }
                 double cs, sn, r;
                 d_lartgf(sH[(l - 1) + ld * (l - 2)], sH[l + ld * (l - 2)], cs, sn, r);
                 sH[(l - 1) + ld * (l - 2)] = r; sH[l + ld * (l - 2)] = 0.0;
-                B[6] = cs; B[7] = sn;
-            }
-            if (kind == 1 || kind == 3) {
-                // the right reflector, from the 3x3 block of T after this bulge's left reflector
-                // (the other bulges' left reflectors act on other rows)
-                const int k = c + 1;
-                const double v1 = B[0], v2 = B[1], tau = B[2];
-                double r1[3], r2[3];
-                for (int jj = 0; jj < 3; ++jj) {
-                    const double *tc = sT + k + ld * (k + jj);
-                    const double t0 = tc[0], t1 = tc[1], t2 = tc[2];
-                    const double sj = tau * (t0 + v1 * t1 + v2 * t2);
-                    r1[jj] = t1 - sj * v1; r2[jj] = t2 - sj * v2;
-                }
-                double wv[3], tau2;
-                d_nullrefl3(r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], wv, tau2);
-                B[3] = wv[1]; B[4] = wv[2]; B[5] = tau2;
-            } else if (kind == 2) {
                 // the closing column rotation, from T(l-1:l, l-1:l) after the row rotation
-                const int l = w - 1;
-                const double cs = B[6], sn = B[7];
-                const double x0 = sT[(l - 1) + ld * (l - 1)], y0 = sT[l + ld * (l - 1)];
-                const double x1 = sT[(l - 1) + ld * l], y1 = sT[l + ld * l];
-                double c2, s2, r;
-                d_lartgf(cs * y1 - sn * x1, cs * y0 - sn * x0, c2, s2, r);
-                B[3] = c2; B[4] = s2; B[5] = r;
+                double c2, s2, r2;
+                d_lartgf(cs * yy1 - sn * xx1, cs * y0 - sn * x0, c2, s2, r2);
+                B[6] = cs; B[7] = sn; B[3] = c2; B[4] = s2; B[5] = r2;
             }
             bc[b] = c; bk[b] = kind;
         }
