@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--ktime", type=int, default=1, help="record per-class kernel event times")
+    ap.add_argument("--ktime-every", type=int, default=16,
+                    help="time one launch in k (pseudo-random) of each kernel class (events cost queue packets)")
     ap.add_argument("--workload", choices=["qz", "batched"], default="qz")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tune", type=int, default=1, help="run the batched autotuner first")
@@ -191,6 +193,8 @@ def main():
     L = gqz.init(0)
     L.gqz_ktime_enable.argtypes = [C.c_uint]
     L.gqz_ktime_get.argtypes = [C.POINTER(KTime), C.c_int]
+    L.gqz_ktime_sample.argtypes = [C.c_int]
+    L.gqz_ktime_sample(args.ktime_every)
     n = args.n
     prm = gqz.GqzParams()
     if args.params == "learned":

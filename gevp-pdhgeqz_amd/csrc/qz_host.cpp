@@ -126,6 +126,12 @@ struct KTimer {
     // classes 0..GQZ_KC_COUNT-1: top level; +GQZ_KC_COUNT: inside AED windows (nested levels)
     long launches[2 * GQZ_KC_COUNT] = {};
     double ms[2 * GQZ_KC_COUNT] = {}, flops[2 * GQZ_KC_COUNT] = {}, bytes[2 * GQZ_KC_COUNT] = {};
+    // one launch in `every` is timed, chosen by a linear congruential generator (a fixed stride
+    // could lock onto the period of the launch sequence, e.g. left / right slab updates); events
+    // add queue packets and host calls.  `timed` counts the timed launches, ms sums their times
+    int every = 1;
+    unsigned long long rng = 1;
+    long timed[2 * GQZ_KC_COUNT] = {};
     int level = 0;
 } KT;
 
@@ -142,17 +148,28 @@ hipEvent_t kt_event() {
     return e;
 }
 
-// returns a slot (or -1 when the class is not timed); the matching kt_end records the stop event
+// returns a slot (or -1 when the class is not timed); the matching kt_end records the stop event.
+// Slots <= -2 stand for an untimed launch of class -slot - 2: only its work is counted.
 int kt_begin(int cls) {
     if (!(KT.mask & (1u << cls))) return -1;
-    KTimer::Pend p = { cls + (KT.level > 0 ? GQZ_KC_COUNT : 0), kt_event(), kt_event(), 0.0, 0.0 };
+    const int c = cls + (KT.level > 0 ? GQZ_KC_COUNT : 0);
+    if (KT.every > 1) {
+        KT.rng = KT.rng * 6364136223846793005ULL + 1442695040888963407ULL;
+        if ((KT.rng >> 33) % (unsigned long long)KT.every != 0) return -2 - c;
+    }
+    KTimer::Pend p = { c, kt_event(), kt_event(), 0.0, 0.0 };
     HIPCHK(hipEventRecord(p.e0, L.st));
     KT.pend.push_back(p);
     return (int)KT.pend.size() - 1;
 }
 
 void kt_end(int slot, double flops, double bytes) {
-    if (slot < 0) return;
+    if (slot == -1) return;
+    if (slot < -1) {
+        const int c = -slot - 2;
+        KT.launches[c]++; KT.flops[c] += flops; KT.bytes[c] += bytes;
+        return;
+    }
     KTimer::Pend &p = KT.pend[slot];
     p.flops = flops; p.bytes = bytes;
     HIPCHK(hipEventRecord(p.e1, L.st));
@@ -164,6 +181,7 @@ void kt_drain() {
         float t = 0.f;
         HIPCHK(hipEventElapsedTime(&t, p.e0, p.e1));
         KT.launches[p.cls]++; KT.ms[p.cls] += t; KT.flops[p.cls] += p.flops; KT.bytes[p.cls] += p.bytes;
+        KT.timed[p.cls]++;
         KT.pool.push_back(p.e0); KT.pool.push_back(p.e1);
     }
     KT.pend.clear();
@@ -744,7 +762,17 @@ int gqz_ktime_enable(unsigned mask) {
         kt_drain();
     }
     KT.mask = mask;
-    for (int c = 0; c < 2 * GQZ_KC_COUNT; ++c) { KT.launches[c] = 0; KT.ms[c] = KT.flops[c] = KT.bytes[c] = 0.0; }
+    KT.rng = 1;
+    for (int c = 0; c < 2 * GQZ_KC_COUNT; ++c) {
+        KT.launches[c] = KT.timed[c] = 0;
+        KT.ms[c] = KT.flops[c] = KT.bytes[c] = 0.0;
+    }
+    return 0;
+}
+
+int gqz_ktime_sample(int every) {
+    if (every < 1) return -1;
+    KT.every = every;
     return 0;
 }
 
@@ -759,7 +787,9 @@ int gqz_ktime_get(gqz_ktime *out, int cap) {
     }
     int k = std::min(cap, (int)(2 * GQZ_KC_COUNT));
     for (int c = 0; c < k; ++c) {
-        out[c].name = names[c]; out[c].launches = KT.launches[c]; out[c].ms = KT.ms[c];
+        // summed time of the timed launches, scaled to all launches of the class
+        out[c].name = names[c]; out[c].launches = KT.launches[c];
+        out[c].ms = KT.timed[c] ? KT.ms[c] * (double)KT.launches[c] / (double)KT.timed[c] : 0.0;
         out[c].flops = KT.flops[c]; out[c].bytes = KT.bytes[c];
     }
     return k;

@@ -161,6 +161,9 @@ typedef struct {
     double ms, flops, bytes;
 } gqz_ktime;
 int gqz_ktime_enable(unsigned mask);          /* resets the totals */
+/* time one launch in `every` (pseudo-random choice; default 1: all); ms is then the mean event time
+ * of the timed launches times the number of launches, flops and bytes still cover every launch */
+int gqz_ktime_sample(int every);
 /* synchronises; returns the number of entries (2 * GQZ_KC_COUNT: each class at the top level, then
  * inside AED windows, named "<class>@aed") */
 int gqz_ktime_get(gqz_ktime *out, int cap);
