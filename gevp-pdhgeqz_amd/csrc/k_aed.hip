@@ -650,8 +650,10 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
     }
     aed_reorder_body<AEDS_MAXW>(sH, sT, sU, sV, ld, jw, s, smlnum, pscr, res);
     aed_eigs_body(sH, sT, ld, jw, war, wai, wb, tid, nt);
-    const int kbc = res[0];
-    if (kbc < jw || s == 0.0) {
+    // a rejected swap (DTGEX2 INFO = 1) leaves the window's factors unreliable: the AED is then
+    // discarded (nothing written back, no deflation)
+    const int kbc = res[2] ? jw : res[0];
+    if (!res[2] && (kbc < jw || s == 0.0)) {
         if (kbc > 0 && s != 0.0) aed_spike_body(sH, sT, sU, sV, ld, jw, s, kbc, scr, sv, sw, red);
         if (kbc > 2 && s != 0.0) wg_gghrd(sH, sT, ld, jw, 0, kbc - 1, sU, sV, ld, jw, sh);
         __syncthreads();
@@ -664,7 +666,7 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
         }
         if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = s * sU[0];
     }
-    if (tid == 0) { out[0] = res[0]; out[1] = res[1]; out[2] = res[2]; out[3] = res[3]; out[4] = 0; }
+    if (tid == 0) { out[0] = kbc; out[1] = res[1]; out[2] = res[2]; out[3] = res[2] ? 1 : res[3]; out[4] = 0; }
 }
 
 void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwtop, int jw, int has_spike,

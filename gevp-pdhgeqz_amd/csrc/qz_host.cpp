@@ -308,6 +308,12 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         R.st.swaps += L.h_out[17];
         R.st.swap_fail += L.h_out[18] ? 1 : 0;
         const int spike = L.h_out[19];
+        if (L.h_out[18]) {          // rejected swap: the kernel wrote nothing back; no shifts either
+            L.ws.release(mk);
+            ls = 0;
+            if (level == 0) R.st.t_aed += now() - t0;
+            return 0;
+        }
         if (kbc < jw || !spike) apply_window(R, v, kwtop, jw, U, V);
         sh.assign(3 * (size_t)kbc, 0.0);
         for (int i = 0; i < kbc; ++i) {
@@ -357,6 +363,16 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     R.st.swaps += L.h_out[17];
     R.st.swap_fail += L.h_out[18] ? 1 : 0;
     int spike = L.h_out[19];
+    if (L.h_out[18]) {
+        // a rejected swap (DTGEX2 INFO = 1) leaves the window's factors unreliable: the AED is
+        // discarded (the window copy is not written back); no deflation and no shifts, so the
+        // sweep falls back to the eigenvalues of the bottom 2x2 block
+        L.ws.release(mk);
+        ls = 0;
+        R.st.aed_calls++;
+        if (level == 0) R.st.t_aed += now() - t0;
+        return 0;
+    }
     // nothing deflated: the window is left as it was (PDHGEQZ3.f:840, :1555); its eigenvalues are
     // still returned as shifts
     const bool restore = (kbc < jw) || !spike;
