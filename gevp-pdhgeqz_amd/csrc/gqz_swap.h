@@ -126,11 +126,19 @@ __device__ __forceinline__ void r_qr_full(double (&X)[M * KC], double (&Qo)[M * 
         double tau = 0.0, beta = alpha, sc = 0.0;
         if (ss != 0.0) {
             beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-            tau = (beta - alpha) / beta;
             sc = 1.0 / (alpha - beta);
         }
+        // tau = 2 / (v^T v) of the stored v, so that the reflector is orthogonal whatever the
+        // rounding of v: (beta - alpha) / beta equals it only in exact arithmetic, and with the
+        // squares in the subnormal range the two differ by many percent (a collapsed column of the
+        // swap factor, accepted by the stability tests when the block is nearly singular)
+        double vv = 1.0;
 #pragma unroll
-        for (int i = 0; i < M; ++i) vs[j][i] = (i < j) ? 0.0 : (i == j ? 1.0 : X[i + M * j] * sc);
+        for (int i = 0; i < M; ++i) {
+            vs[j][i] = (i < j) ? 0.0 : (i == j ? 1.0 : X[i + M * j] * sc);
+            if (i > j) vv += vs[j][i] * vs[j][i];
+        }
+        if (ss != 0.0) tau = 2.0 / vv;
         taus[j] = tau;
         X[j + M * j] = beta;
 #pragma unroll

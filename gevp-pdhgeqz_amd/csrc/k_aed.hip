@@ -64,7 +64,13 @@ __device__ void aed_reorder_body(double *Hw, double *Tw, double *U, double *V, i
     }
     if (tid == 0) { s_kb = jw - 1; s_ilst = 0; s_knt = 0; s_fail = 0; s_swaps = 0; s_np = 0; }
     __syncthreads();
-    for (int round = 0; round < 4 * jw + 8; ++round) {
+    // a window that is not quasi-triangular (two consecutive non-zero subdiagonal entries: an
+    // unreduced 3x3 or larger block) has no valid block structure: no reordering, reported as
+    // a rejected swap (the caller then discards the AED)
+    for (int i = 1 + tid; i + 1 < jw; i += nt)
+        if (bsz[i] == 0 && bsz[i + 1] == 0) s_fail = 1;
+    __syncthreads();
+    for (int round = 0; round < (s_fail ? 0 : 4 * jw + 8); ++round) {
         if (tid == 0) {
             // ---- test phase: examine blocks at the bottom while they are unexamined ----
             while (s_kb >= 0 && s_knt < jw) {
@@ -254,7 +260,11 @@ extern "C" __global__ void __launch_bounds__(COOP_NT) k_aed_reorder_coop(double 
     }
     if (tid == 0) { s_kb = jw - 1; s_ilst = 0; s_knt = 0; s_fail = 0; s_swaps = 0; s_np = 0; }
     __syncthreads();
-    for (int round = 0; round < 4 * jw + 8; ++round) {
+    // not quasi-triangular: no reordering (every workgroup sees the same window)
+    for (int i = 1 + tid; i + 1 < jw; i += nt)
+        if (bsz[i] == 0 && bsz[i + 1] == 0) s_fail = 1;
+    __syncthreads();
+    for (int round = 0; round < (s_fail ? 0 : 4 * jw + 8); ++round) {
         if (tid == 0) {
             while (s_kb >= 0 && s_knt < jw) {
                 const int kb = s_kb;
