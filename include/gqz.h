@@ -150,9 +150,10 @@ int gqz_mlp_train_step_dev(float *params, float *grad, float *m, float *v, int s
 int gqz_mlp_policy(const float *params_host, int enable);
 
 /* ---- kernel timing (no counterpart in the reference) -----------------------------------------
- * HIP events recorded on the library stream around every launch of the enabled kernel classes
- * (bit c of mask = class c); per class: launches, summed event time and the algorithmic work of
- * those launches (flops; bytes = each operand read once and each result written once). */
+ * HIP events recorded on the library stream around the launches of the enabled kernel classes
+ * (bit c of mask = class c; every launch, or one in k with gqz_ktime_sample); per class: launches,
+ * event time and the algorithmic work of those launches (flops; bytes = each operand read once and
+ * each result written once). */
 enum { GQZ_KC_SLAB = 0, GQZ_KC_CHASE, GQZ_KC_SMALL, GQZ_KC_REORDER, GQZ_KC_SPIKE, GQZ_KC_HT, GQZ_KC_INF,
        GQZ_KC_COUNT };
 typedef struct {
