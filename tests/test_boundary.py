@@ -133,3 +133,27 @@ def test_no_gpu_fails_loudly(lib):
     assert info == -100
     lib.gqz_last_error.restype = C.c_char_p
     assert b"GPU" in lib.gqz_last_error() or b"HIP" in lib.gqz_last_error() or b"hip" in lib.gqz_last_error()
+
+
+def test_bench_traffic_summary_is_consistent():
+    """The counter summary that bench.py reads for roofline.traffic covers the kernel classes it
+    names, with calibration factors from kernels of known size."""
+    import json
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    with open(bench.TRAFFIC_FILE) as f:
+        t = json.load(f)
+    assert t["n"] == 16000 and t["params"] == "learned"
+    assert 1.9 < t["calibration"]["fetch_factor"] < 2.1 and 0.99 < t["calibration"]["write_factor"] < 1.01
+    for cls, kernels in t["classes"].items():
+        for k in kernels:
+            if k in t["kernels"]:
+                e = t["kernels"][k]
+                assert e["dispatches"] > 0 and e["fetch_kib"] >= 0 and e["write_kib"] >= 0
+    # chase windows: one kernel, traffic = calibrated FETCH + WRITE per dispatch
+    e = t["kernels"]["k_chase"]
+    want = 1024.0 * (e["fetch_kib"] * t["calibration"]["fetch_factor"] + e["write_kib"] * t["calibration"]["write_factor"])
+    assert abs(bench.class_traffic("chase_window", 16000) - want) < 1e-6 * want
+    assert bench.class_traffic("chase_window", 8000) is None       # other workloads: not covered
