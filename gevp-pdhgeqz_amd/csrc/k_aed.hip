@@ -674,7 +674,10 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
             U[i + (size_t)jw * j] = sU[i + ld * j];
             V[i + (size_t)jw * j] = sV[i + ld * j];
         }
-        if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = s * sU[0];
+        // the spike after its reflector: s U(0,0) on the first row; all of it negligible when the
+        // whole window deflated (kbc = 0), then dropped: a tiny non-zero entry here would join two
+        // converged blocks across the deflation boundary for good
+        if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = kbc > 0 ? s * sU[0] : 0.0;
     }
     if (tid == 0) { out[0] = kbc; out[1] = res[1]; out[2] = res[2]; out[3] = res[2] ? 1 : res[3]; out[4] = 0; }
 }

@@ -387,7 +387,7 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
             gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
             kt_end(kg, 0.0, 32.0 * jj);
         }
-        gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? 1 : 0);
+        gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? (kbc > 0 ? 1 : 2) : 0);
         apply_window(R, v, kwtop, jw, U, V);
     }
     sh.assign(3 * (size_t)kbc, 0.0);
