@@ -674,8 +674,8 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
             U[i + (size_t)jw * j] = sU[i + ld * j];
             V[i + (size_t)jw * j] = sV[i + ld * j];
         }
-        // the spike after its reflector: s U(0,0) on the first row; all of it negligible when the
-        // whole window deflated (kbc = 0), then dropped: a tiny non-zero entry here would join two
+        // the spike after its reflector: s U(0,0) on the first row; dropped when the whole window
+        // deflated (kbc = 0; PDHGEQZ3.f:831, :1346): a tiny non-zero entry here would join two
         // converged blocks across the deflation boundary for good
         if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = kbc > 0 ? s * sU[0] : 0.0;
     }

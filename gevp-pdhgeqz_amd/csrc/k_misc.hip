@@ -140,7 +140,7 @@ extern "C" __global__ void k_window_out(double *H, double *T, int ld, int k0, in
         H[GQZ_IX(k0 + i, k0 + j, ld)] = Hw[idx];
         T[GQZ_IX(k0 + i, k0 + j, ld)] = Tw[idx];
         // the spike value is read and rewritten by the same thread (i = 0); has_spike = 2: the
-        // whole window deflated, the spike is dropped (DLAQR3 sets S = 0 when NS = 0)
+        // whole window deflated, the spike is dropped (PDHGEQZ3.f:831, :1346: SCAL = 0 when NS = 0)
         if (has_spike && j == 0) {
             if (i == 0) H[GQZ_IX(k0, k0 - 1, ld)] = has_spike == 1 ? H[GQZ_IX(k0, k0 - 1, ld)] * U[0] : 0.0;
             else H[GQZ_IX(k0 + i, k0 - 1, ld)] = 0.0;

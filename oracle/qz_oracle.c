@@ -1226,7 +1226,8 @@ static int aed(qzctx *c, int ktop, int kbot, int jw, double *sr, double *si, dou
         Hc(kwtop + i, kwtop + j) = Hw[i + (size_t)jw * j];
         Tc(kwtop + i, kwtop + j) = Tw[i + (size_t)jw * j];
     }
-    if (kwtop > ktop) Hc(kwtop, kwtop - 1) = s * U[0];
+    /* the spike's first entry; zero when the whole window deflated (PDHGEQZ3.f:831, :1346) */
+    if (kwtop > ktop) Hc(kwtop, kwtop - 1) = kbc > 0 ? s * U[0] : 0.0;
     if (kwtop > ktop) for (int i = 1; i < jw; ++i) Hc(kwtop + i, kwtop - 1) = 0.0;
     int ncol = c->n - (kbot + 1);
     apply_left_t(&Hc(kwtop, kbot + 1), c->ldh, jw, ncol, U, jw, wk);
