@@ -8,9 +8,10 @@
 //   d_lartg  ~ DLARTG (used e.g. at PDHGEQZB.f:373-406)
 //   d_larfg  ~ DLARFG (DHGEQZ5.f:135)
 //   d_shift_col ~ QZFCOL (src/kkqz/qzfcol.f:1-116), homogeneous in (alpha, beta)
-//   d_invhse3   ~ INVHSE with ITOPP=2 (src/kkqz/invhse.f:61-71)
+//   d_invhse3f  ~ INVHSE with ITOPP=2 (src/kkqz/invhse.f:61-71); d_nullrefl3: the same reflector
+//                 from the cross product of the two rows when they are well apart
 //   d_lagv2     ~ DLAGV2 (PDHGEQZ0.f:622-674)
-//   d_tgex2     ~ DTGEX2 (src/reorder/dtgex2.f), direct swap via the generalized Sylvester eq.
+//   r_tgex2 (gqz_swap.h) ~ DTGEX2 (src/reorder/dtgex2.f), direct swap via the generalized Sylvester eq.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -180,41 +181,12 @@ __device__ inline void d_shift_col(const double *H, int ldh, const double *T, in
     if (m > 0.0 && isfinite(m)) { v[0] /= m; v[1] /= m; v[2] /= m; }
 }
 
-// Opposite Householder of a 3x3 block of T: its first column spans the vector orthogonal to
-// rows 1 and 2 (first row of the orthogonal factor of an RQ decomposition).
-__device__ inline void d_invhse3(const double *T, int ldt, double v[3], double &tau) {
-    double t[9];
-    for (int j = 0; j < 3; ++j)
-        for (int i = 0; i < 3; ++i) t[i + 3 * j] = T[GQZ_IX(i, j, ldt)];
-    double a = t[8];
-    double x[2] = { t[2], t[5] };
-    double tau1;
-    d_larfg3(a, x[0], x[1], tau1);
-    double u1[3] = { x[0], x[1], 1.0 };
-    for (int i = 0; i < 3; ++i) {
-        double s = tau1 * (t[i] * u1[0] + t[i + 3] * u1[1] + t[i + 6] * u1[2]);
-        t[i] -= s * u1[0]; t[i + 3] -= s * u1[1]; t[i + 6] -= s * u1[2];
-    }
-    double a2 = t[4];
-    double x2[1] = { t[1] };
-    double tau2;
-    d_larfg2(a2, x2[0], tau2);
-    double w0 = 1.0, w1 = 0.0, w2 = 0.0;
-    double s2 = tau2 * (w0 * x2[0] + w1);
-    w0 -= s2 * x2[0]; w1 -= s2;
-    double s1 = tau1 * (w0 * u1[0] + w1 * u1[1] + w2);
-    w0 -= s1 * u1[0]; w1 -= s1 * u1[1]; w2 -= s1;
-    // (w0, w1, w2) is a unit vector: reflect it onto -sign(w0) e0 with v = w + sign(w0) e0 scaled to
-    // v0 = 1 and tau = 2 / (v^T v) from v itself; a norm-based DLARFG on a unit vector rounds the norm
-    // to 1 with a bias that makes the accumulated Z lose orthogonality coherently
-    double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
-    double p1 = w1 / p0, p2 = w2 / p0;
-    tau = 2.0 / (1.0 + p1 * p1 + p2 * p2);
-    v[0] = 1.0; v[1] = p1; v[2] = p2;
-}
-
-// d_invhse3 from rows 1 and 2 of the block only (the same two reflectors and the same final
-// reflector), with the short-latency primitives
+// Opposite Householder of a 3x3 block of T (INVHSE): its first column spans the vector orthogonal
+// to rows 1 and 2 (first row of the orthogonal factor of an RQ decomposition), built from rows 1
+// and 2 only with the short-latency primitives.  The reflector maps the unit vector w onto
+// -sign(w0) e0 with v = w + sign(w0) e0 scaled to v0 = 1 and tau = 2 / (v^T v) from v itself (a
+// norm-based DLARFG on a unit vector rounds the norm to 1 with a bias that makes the accumulated Z
+// lose orthogonality coherently).
 __device__ __forceinline__ void d_invhse3f(double r10, double r11, double r12, double r20, double r21, double r22,
                                            double v[3], double &tau) {
     double a = r22, x0 = r20, x1 = r21, tau1;
