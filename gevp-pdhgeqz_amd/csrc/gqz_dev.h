@@ -229,16 +229,7 @@ __device__ __forceinline__ void d_nullrefl3(double r10, double r11, double r12, 
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
-// small dense helpers on column-major m x m (m <= 4) arrays
-__device__ inline void d_mm(int m, const double *A, int ta, const double *B, int tb, double *C) {
-    for (int j = 0; j < m; ++j)
-        for (int i = 0; i < m; ++i) {
-            double s = 0.0;
-            for (int k = 0; k < m; ++k)
-                s += (ta ? A[k + m * i] : A[i + m * k]) * (tb ? B[j + m * k] : B[k + m * j]);
-            C[i + m * j] = s;
-        }
-}
+// 2x2 transforms of two rows / two columns of small column-major arrays
 __device__ inline void d_g2_rows(double *M, int ld, int i, int c0, int c1, const double G[4]) {
     for (int j = c0; j < c1; ++j) {
         double x = M[GQZ_IX(i, j, ld)], y = M[GQZ_IX(i + 1, j, ld)];
@@ -333,157 +324,3 @@ __device__ inline void d_pair_eigs(const double *A, const double *B, double ar[2
     }
 }
 
-// ---- DTGEX2: swap adjacent blocks (n1, n2 in {1,2}) of the m x m corner (S, T) -------------
-// On success returns 0 and Qa, Za (m x m) with S_new = Qa^T S Za, T_new likewise (written back
-// into S, T with the lower-left block and T's lower part exactly zero).  Returns 1 when the
-// weak or strong stability test rejects the swap.
-__device__ inline double d_solve_cp(int m, double *K, double *rhs) {
-    int jpiv[8];
-    double xmax = 0.0;
-    for (int i = 0; i < m * m; ++i) xmax = fmax(xmax, fabs(K[i]));
-    double smlnum = GQZ_SAFMIN / GQZ_ULP;
-    double smin = fmax(GQZ_ULP * xmax, smlnum);
-    for (int i = 0; i < m; ++i) jpiv[i] = i;
-    for (int k = 0; k < m; ++k) {
-        int pi = k, pj = k; double pv = -1.0;
-        for (int j = k; j < m; ++j)
-            for (int i = k; i < m; ++i)
-                if (fabs(K[i + m * j]) > pv) { pv = fabs(K[i + m * j]); pi = i; pj = j; }
-        if (pi != k) {
-            for (int j = 0; j < m; ++j) { double t = K[k + m * j]; K[k + m * j] = K[pi + m * j]; K[pi + m * j] = t; }
-            double t = rhs[k]; rhs[k] = rhs[pi]; rhs[pi] = t;
-        }
-        if (pj != k) {
-            for (int i = 0; i < m; ++i) { double t = K[i + m * k]; K[i + m * k] = K[i + m * pj]; K[i + m * pj] = t; }
-            int t = jpiv[k]; jpiv[k] = jpiv[pj]; jpiv[pj] = t;
-        }
-        if (fabs(K[k + m * k]) < smin) K[k + m * k] = smin;
-        for (int i = k + 1; i < m; ++i) {
-            double f = K[i + m * k] / K[k + m * k];
-            K[i + m * k] = f;
-            for (int j = k + 1; j < m; ++j) K[i + m * j] -= f * K[k + m * j];
-            rhs[i] -= f * rhs[k];
-        }
-    }
-    double scale = 1.0, bmax = 0.0;
-    for (int i = 0; i < m; ++i) bmax = fmax(bmax, fabs(rhs[i]));
-    if (2.0 * smlnum * bmax > fabs(K[(m - 1) + m * (m - 1)])) {
-        double t = 0.5 / bmax;
-        for (int i = 0; i < m; ++i) rhs[i] *= t;
-        scale = t;
-    }
-    double x[8];
-    for (int i = m - 1; i >= 0; --i) {
-        double s = rhs[i];
-        for (int j = i + 1; j < m; ++j) s -= K[i + m * j] * x[j];
-        x[i] = s / K[i + m * i];
-    }
-    for (int i = 0; i < m; ++i) rhs[jpiv[i]] = x[i];
-    return scale;
-}
-
-__device__ inline void d_qr_full(int m, int k, double *X, double *Qo) {
-    double vs[4][4], taus[4];
-    for (int j = 0; j < k; ++j) {
-        double a = X[j + m * j];
-        double x[4];
-        for (int i = j + 1; i < m; ++i) x[i - j - 1] = X[i + m * j];
-        d_larfg(m - j, a, x, 1, taus[j]);
-        X[j + m * j] = a;
-        vs[j][j] = 1.0;
-        for (int i = j + 1; i < m; ++i) { vs[j][i] = x[i - j - 1]; X[i + m * j] = 0.0; }
-        for (int jj = j + 1; jj < k; ++jj) {
-            double s = 0.0;
-            for (int i = j; i < m; ++i) s += vs[j][i] * X[i + m * jj];
-            s *= taus[j];
-            for (int i = j; i < m; ++i) X[i + m * jj] -= s * vs[j][i];
-        }
-    }
-    for (int j = 0; j < m; ++j)
-        for (int i = 0; i < m; ++i) Qo[i + m * j] = (i == j) ? 1.0 : 0.0;
-    for (int j = k - 1; j >= 0; --j)
-        for (int jj = 0; jj < m; ++jj) {
-            double s = 0.0;
-            for (int i = j; i < m; ++i) s += vs[j][i] * Qo[i + m * jj];
-            s *= taus[j];
-            for (int i = j; i < m; ++i) Qo[i + m * jj] -= s * vs[j][i];
-        }
-}
-
-__device__ inline int d_tgex2(int m, int n1, int n2, double *S, double *T, double *Qa, double *Za) {
-    double S0[16], T0[16];
-    double sa = 0.0, sb = 0.0;
-    for (int i = 0; i < m * m; ++i) { S0[i] = S[i]; T0[i] = T[i]; sa += S[i] * S[i]; sb += T[i] * T[i]; }
-    double smlnum = GQZ_SAFMIN / GQZ_ULP;
-    double tha = fmax(20.0 * GQZ_ULP * sqrt(sa), smlnum), thb = fmax(20.0 * GQZ_ULP * sqrt(sb), smlnum);
-    int nn = n1 * n2, m2 = 2 * nn;
-    double K[64], rhs[8];
-    for (int i = 0; i < 64; ++i) K[i] = 0.0;
-    for (int q = 0; q < n2; ++q)
-        for (int p = 0; p < n1; ++p) {
-            int row = p + n1 * q;
-            for (int k = 0; k < n1; ++k) {
-                K[row + m2 * (k + n1 * q)] += S0[p + m * k];
-                K[row + nn + m2 * (k + n1 * q)] += T0[p + m * k];
-            }
-            for (int k = 0; k < n2; ++k) {
-                K[row + m2 * (nn + p + n1 * k)] -= S0[(n1 + k) + m * (n1 + q)];
-                K[row + nn + m2 * (nn + p + n1 * k)] -= T0[(n1 + k) + m * (n1 + q)];
-            }
-            rhs[row] = S0[p + m * (n1 + q)];
-            rhs[row + nn] = T0[p + m * (n1 + q)];
-        }
-    double g = d_solve_cp(m2, K, rhs);
-    double XR[16], XL[16], W[16], S2[16], T2[16];
-    for (int j = 0; j < n2; ++j)
-        for (int i = 0; i < m; ++i) {
-            XR[i + m * j] = (i < n1) ? -rhs[i + n1 * j] : ((i - n1) == j ? g : 0.0);
-            XL[i + m * j] = (i < n1) ? -rhs[nn + i + n1 * j] : ((i - n1) == j ? g : 0.0);
-        }
-    d_qr_full(m, n2, XR, Za);
-    d_qr_full(m, n2, XL, Qa);
-    d_mm(m, Qa, 1, S0, 0, W); d_mm(m, W, 0, Za, 0, S2);
-    d_mm(m, Qa, 1, T0, 0, W); d_mm(m, W, 0, Za, 0, T2);
-    double ea = 0.0, eb = 0.0;
-    for (int j = 0; j < n2; ++j)
-        for (int i = n2; i < m; ++i) { ea += S2[i + m * j] * S2[i + m * j]; eb += T2[i + m * j] * T2[i + m * j]; }
-    if (!(sqrt(ea) <= tha && sqrt(eb) <= thb)) return 1;
-    for (int j = 0; j < n2; ++j)
-        for (int i = n2; i < m; ++i) { S2[i + m * j] = 0.0; T2[i + m * j] = 0.0; }
-    int off[2] = { 0, n2 }, sz[2] = { n2, n1 };
-    for (int b = 0; b < 2; ++b) {
-        int o = off[b];
-        if (sz[b] != 2) continue;
-        double c, s, r;
-        d_lartg(T2[(o + 1) + m * (o + 1)], T2[(o + 1) + m * o], c, s, r);
-        for (int i = 0; i < m; ++i) {     // columns (o+1, o) rotation
-            double x, y;
-            x = S2[i + m * (o + 1)]; y = S2[i + m * o]; S2[i + m * (o + 1)] = c * x + s * y; S2[i + m * o] = c * y - s * x;
-            x = T2[i + m * (o + 1)]; y = T2[i + m * o]; T2[i + m * (o + 1)] = c * x + s * y; T2[i + m * o] = c * y - s * x;
-            x = Za[i + m * (o + 1)]; y = Za[i + m * o]; Za[i + m * (o + 1)] = c * x + s * y; Za[i + m * o] = c * y - s * x;
-        }
-        T2[(o + 1) + m * o] = 0.0;
-        double Ab[4] = { S2[o + m * o], S2[(o + 1) + m * o], S2[o + m * (o + 1)], S2[(o + 1) + m * (o + 1)] };
-        double Bb[4] = { T2[o + m * o], 0.0, T2[o + m * (o + 1)], T2[(o + 1) + m * (o + 1)] };
-        double QL[4], ZR[4];
-        d_lagv2(Ab, Bb, QL, ZR);
-        d_g2_rows(S2, m, o, 0, m, QL);
-        d_g2_rows(T2, m, o, 0, m, QL);
-        d_g2_cols(S2, m, o, 0, m, ZR);
-        d_g2_cols(T2, m, o, 0, m, ZR);
-        d_g2_cols(Qa, m, o, 0, m, QL);
-        d_g2_cols(Za, m, o, 0, m, ZR);
-        S2[o + m * o] = Ab[0]; S2[(o + 1) + m * o] = Ab[1]; S2[o + m * (o + 1)] = Ab[2]; S2[(o + 1) + m * (o + 1)] = Ab[3];
-        T2[o + m * o] = Bb[0]; T2[(o + 1) + m * o] = 0.0; T2[o + m * (o + 1)] = Bb[2]; T2[(o + 1) + m * (o + 1)] = Bb[3];
-    }
-    double S3[16], T3[16];
-    d_mm(m, Qa, 0, S2, 0, W); d_mm(m, W, 0, Za, 1, S3);
-    d_mm(m, Qa, 0, T2, 0, W); d_mm(m, W, 0, Za, 1, T3);
-    double da = 0.0, db = 0.0;
-    for (int i = 0; i < m * m; ++i) { da += (S3[i] - S0[i]) * (S3[i] - S0[i]); db += (T3[i] - T0[i]) * (T3[i] - T0[i]); }
-    if (!(sqrt(da) <= tha && sqrt(db) <= thb)) return 1;
-    for (int j = 0; j < m; ++j)
-        for (int i = j + 1; i < m; ++i) T2[i + m * j] = 0.0;
-    for (int i = 0; i < m * m; ++i) { S[i] = S2[i]; T[i] = T2[i]; }
-    return 0;
-}
