@@ -262,3 +262,81 @@ def test_mlp_oracle_forward_closed_form():
     expect = W3 @ a2 + P[5184:5187]
     Y = O.mlp_forward(P, X)
     assert np.allclose(Y, expect[None, :], rtol=0, atol=1e-13)
+
+
+def _max_matched_rel_err(got, ref):
+    """Largest |got - ref| / (1 + |ref|) over the best one-to-one matching of two eigenvalue lists
+    (sorting alone can split a conjugate pair whose real parts differ in the last bit)."""
+    from scipy.optimize import linear_sum_assignment
+    D = np.abs(got[:, None] - ref[None, :]) / (1.0 + np.abs(ref[None, :]))
+    ri, ci = linear_sum_assignment(D)
+    return D[ri, ci].max()
+
+
+def _mp_gen_eigs(A, B, dps=50):
+    """Roots of det(A - l B) by brute force in 50-digit arithmetic: the degree-n polynomial is
+    interpolated from n + 1 exact-enough determinant values and solved with mpmath.polyroots."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    n = A.shape[0]
+    Am, Bm = mp.matrix(A.tolist()), mp.matrix(B.tolist())
+    xs = [mp.mpf(k) for k in range(n + 1)]
+    ys = [mp.det(Am - x * Bm) for x in xs]
+    # Newton divided differences -> monomial coefficients (highest degree first)
+    c = list(ys)
+    for j in range(1, n + 1):
+        for i in range(n, j - 1, -1):
+            c[i] = (c[i] - c[i - 1]) / (xs[i] - xs[i - j])
+    poly = [mp.mpf(0)] * (n + 1)      # ascending powers
+    poly[0] = c[n]
+    for k in range(n - 1, -1, -1):    # poly <- poly * (x - xs[k]) + c[k]
+        new = [mp.mpf(0)] * (n + 1)
+        for i in range(n):
+            new[i + 1] += poly[i]
+            new[i] -= xs[k] * poly[i]
+        new[0] += c[k]
+        poly = new
+    roots = mp.polyroots(list(reversed(poly)), maxsteps=200, extraprec=200)
+    return np.array([complex(r) for r in roots])
+
+
+@pytest.mark.parametrize("n,seed", [(3, 1), (4, 2), (5, 3), (6, 4), (6, 5)])
+def test_oracle_tiny_pencils_brute_force(n, seed):
+    """n <= 6: the oracle's eigenvalues equal the roots of det(H - l T) found by brute force in
+    50-digit arithmetic (no LAPACK involved)."""
+    H, T = Q.pencil_ht_random(n, seed)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T)
+    assert info == 0 and (be > 0).all()
+    assert _max_matched_rel_err((ar + 1j * ai) / be, _mp_gen_eigs(H, T)) < 1e-9
+
+
+def test_oracle_triangular_pencil_is_exact():
+    """A pencil that is already upper triangular deflates at once: alpha_j = +-H(j,j) and
+    beta_j = |T(j,j)| exactly, in diagonal order, with beta = 0 exactly where T(j,j) = 0."""
+    n = 30
+    rng = np.random.default_rng(11)
+    H = np.triu(rng.standard_normal((n, n)))
+    T = np.triu(rng.standard_normal((n, n)))
+    T[[4, 17], [4, 17]] = 0.0
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=1, compz=1)
+    assert info == 0
+    assert np.all(ai == 0.0)
+    np.testing.assert_array_equal(be, np.abs(np.diag(T)))
+    fin = np.diag(T) != 0
+    np.testing.assert_array_equal(ar[fin], np.sign(np.diag(T))[fin] * np.diag(H)[fin])
+    assert int((be == 0).sum()) == 2
+
+
+def test_oracle_standard_problem_b_identity():
+    """B = I reduces the pencil to the standard eigenproblem: the oracle's eigenvalues equal
+    LAPACK DHSEQR's (numpy eigvals) of the Hessenberg H within a condition-weighted bound."""
+    n = 80
+    H, _ = Q.pencil_ht_random(n, 21)
+    T = np.eye(n)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=1, compz=1)
+    assert info == 0
+    ra, rb, rq, rz = Q.compres(H, T, S, P, Q2, Z2)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+    ratio, _, _ = Q.match_eigs(ar, ai, be, H, T)
+    assert ratio < 1.0
+    assert _max_matched_rel_err((ar + 1j * ai) / be, np.linalg.eigvals(H)) < 1e-8
