@@ -6,7 +6,9 @@
 //   returning the optimum in DWORK(1) and IWORK(1) (PDHGEQZ.f:409-430).  The reported optimum is
 //   that of PDHGEQZ1 (N*NB doubles, 2N integers, PDHGEQZ1.f:369-372): the device workspace is
 //   internal, so the caller's DWORK/IWORK are only checked for size.
-// gqz_pdgghrd_host mirrors PDGGHRD (PDGGHRD.f:13-14, 300-336).  Its Q follows the routine's
+// gqz_pdgghrd_host mirrors PDGGHRD (PDGGHRD.f:13-14, 256-336): LWORK = -1 returns DESCA(MB_) in
+//   WORK(1) before the checks; then -1 COMPQ, -2 COMPZ, -3 N, -4 ILO < 1, -5 IHI outside
+//   [ILO-1, N], -6 MB != NB, -7 LWORK < MB.  An empty range (IHI < ILO + 2) reduces nothing.  Its Q follows the routine's
 //   implemented convention (rotations applied to Q from the left, so the drivers transpose Q
 //   afterwards: testing/EXRAND2.f:346-356): COMPQ='I' returns Q_ht^T and COMPQ='V' returns
 //   Q_ht^T * Q1, where A_out = Q_ht^T A_in Z.
@@ -101,14 +103,15 @@ int gqz_pdhgeqz_host(char job, char compq, char compz, int n, int ilo, int ihi, 
 int gqz_pdgghrd_host(char compq, char compz, int n, int ilo, int ihi, double *A, const int *desca, double *B,
                      const int *descb, double *Q, const int *descq, double *Z, const int *descz, double *work,
                      int lwork) {
+    // the workspace query is answered before any argument check (PDGGHRD.f:256-266)
+    if (lwork == -1) { if (work && desca) work[0] = desca[MB_]; return 0; }
     int icq = decode(compq, "NVI"), icz = decode(compz, "NVI");
     if (icq < 0) return -1;
     if (icz < 0) return -2;
     if (n < 0) return -3;
-    if (ilo < 1 || ilo > std::max(1, n)) return -4;
-    if (ihi < std::min(ilo, n) || ihi > n) return -5;
+    if (ilo < 1) return -4;
+    if (ihi > n || ihi < ilo - 1) return -5;
     if (!desca || desca[NB_] != desca[MB_]) return -6;
-    if (lwork == -1) { if (work) work[0] = desca[MB_]; return 0; }
     if (lwork < desca[MB_]) return -7;
     if (!desc_1x1_ok(desca, n) || !desc_1x1_ok(descb, n)) return -6;
     if (icq > 0 && !desc_1x1_ok(descq, n)) return -6;

@@ -40,6 +40,27 @@ struct GqzError : std::runtime_error {
             throw GqzError(std::string(#x) + " failed: " + hipGetErrorString(e_));              \
     } while (0)
 
+// Device allocation owned by a scope: released when the scope ends, also when a HIP error unwinds
+// it through HIPCHK (stream-ordered when it was allocated on a stream).
+struct DevBuf {
+    double *p = nullptr;
+    hipStream_t st = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    double *alloc(size_t bytes) { HIPCHK(hipMalloc((void **)&p, bytes)); return p; }
+    double *alloc_async(size_t bytes, hipStream_t s) {
+        HIPCHK(hipMallocAsync((void **)&p, bytes, s));
+        st = s;
+        return p;
+    }
+    ~DevBuf() {
+        if (!p) return;
+        if (st) (void)hipFreeAsync(p, st);
+        else (void)hipFree(p);
+    }
+};
+
 struct Arena {
     char *base = nullptr;
     size_t cap = 0, top = 0;
@@ -945,11 +966,10 @@ int gqz_geqrb_dev(char compq, int n, double *A, int lda, double *B, int ldb, dou
     if (n == 0) return 0;
     try {
         if (check_init()) return -100;
-        double *tau = nullptr;
-        HIPCHK(hipMallocAsync((void **)&tau, sizeof(double) * (size_t)n, L.st));
+        DevBuf tau_buf;
+        double *tau = tau_buf.alloc_async(sizeof(double) * (size_t)n, L.st);
         if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
         gqz_launch_house_qr(L.st, n, A, lda, B, ldb, icq ? Q : nullptr, ldq, tau);
-        HIPCHK(hipFreeAsync(tau, L.st));
         HIPCHK(hipStreamSynchronize(L.st));
         return 0;
     } catch (const std::exception &e) {
@@ -964,8 +984,8 @@ int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, in
     if (icq < 0) return -1;
     if (icz < 0) return -2;
     if (n < 0) return -3;
-    if (ilo < 1 || ilo > std::max(1, n)) return -4;
-    if (ihi < std::min(ilo, n) || ihi > n) return -5;
+    if (ilo < 1) return -4;
+    if (ihi > n || ihi < ilo - 1) return -5;
     if (lda < std::max(1, n) || ldb != lda) return -7;
     if ((icq > 0 && icz > 0) && ldq != ldz) return -13;
     if (n == 0) return 0;
@@ -973,8 +993,9 @@ int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, in
         if (check_init()) return -100;
         if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
         if (icz == 1) gqz_launch_identity(L.st, Z, ldz, n, n);
-        gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
-                            icq ? ldq : ldz, n);
+        if (ihi - ilo >= 2)
+            gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
+                                icq ? ldq : ldz, n);
         HIPCHK(hipStreamSynchronize(L.st));
         return 0;
     } catch (const std::exception &e) {
@@ -988,10 +1009,10 @@ int gqz_compres_dev(int n, const double *A, const double *B, const double *S, co
     if (n <= 0) return -1;
     try {
         if (check_init()) return -100;
-        double *W1 = nullptr, *W2 = nullptr;
         size_t nn = (size_t)n * n;
-        HIPCHK(hipMallocAsync((void **)&W1, nn * sizeof(double), L.st));
-        HIPCHK(hipMallocAsync((void **)&W2, nn * sizeof(double), L.st));
+        DevBuf w1_buf, w2_buf;
+        double *W1 = w1_buf.alloc_async(nn * sizeof(double), L.st);
+        double *W2 = w2_buf.alloc_async(nn * sizeof(double), L.st);
         HIPCHK(hipMemsetAsync(L.d_norm, 0, 8 * sizeof(double), L.st));
         const double *X[2] = { A, B }, *Y[2] = { S, T };
         for (int t = 0; t < 2; ++t) {
@@ -1009,8 +1030,6 @@ int gqz_compres_dev(int n, const double *A, const double *B, const double *S, co
         }
         double h[8];
         HIPCHK(hipMemcpyAsync(h, L.d_norm, 8 * sizeof(double), hipMemcpyDeviceToHost, L.st));
-        HIPCHK(hipFreeAsync(W1, L.st));
-        HIPCHK(hipFreeAsync(W2, L.st));
         HIPCHK(hipStreamSynchronize(L.st));
         out[0] = std::sqrt(h[0]) / std::sqrt(h[1]);
         out[1] = std::sqrt(h[2]) / std::sqrt(h[3]);
@@ -1088,8 +1107,8 @@ static int host_roundtrip_hgeqz(char job, char compq, char compz, int n, int ilo
     try {
         if (check_init()) return -100;
         size_t nn = (size_t)n * n;
-        double *d = nullptr;
-        HIPCHK(hipMalloc((void **)&d, sizeof(double) * (4 * nn + 3 * (size_t)n)));
+        DevBuf buf;
+        double *d = buf.alloc(sizeof(double) * (4 * nn + 3 * (size_t)n));
         double *dH = d, *dT = d + nn, *dQ = d + 2 * nn, *dZ = d + 3 * nn, *de = d + 4 * nn;
         HIPCHK(hipMemcpy2DAsync(dH, n * sizeof(double), H, ldh * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpy2DAsync(dT, n * sizeof(double), T, ldt * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
@@ -1107,7 +1126,6 @@ static int host_roundtrip_hgeqz(char job, char compq, char compz, int n, int ilo
             HIPCHK(hipMemcpyAsync(beta, de + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, L.st));
         }
         HIPCHK(hipStreamSynchronize(L.st));
-        HIPCHK(hipFree(d));
         return info;
     } catch (const std::exception &e) {
         L.err = e.what();
@@ -1131,8 +1149,8 @@ int gqz_gghrd(char compq, char compz, int n, int ilo, int ihi, double *A, int ld
     try {
         if (check_init()) return -100;
         size_t nn = (size_t)n * n;
-        double *d = nullptr;
-        HIPCHK(hipMalloc((void **)&d, sizeof(double) * 4 * nn));
+        DevBuf buf;
+        double *d = buf.alloc(sizeof(double) * 4 * nn);
         double *dA = d, *dB = d + nn, *dQ = d + 2 * nn, *dZ = d + 3 * nn;
         HIPCHK(hipMemcpy2DAsync(dA, n * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpy2DAsync(dB, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
@@ -1146,7 +1164,6 @@ int gqz_gghrd(char compq, char compz, int n, int ilo, int ihi, double *A, int ld
             if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
         }
         HIPCHK(hipStreamSynchronize(L.st));
-        HIPCHK(hipFree(d));
         return info;
     } catch (const std::exception &e) {
         L.err = e.what();
@@ -1165,8 +1182,8 @@ int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, i
     try {
         if (check_init()) return -100;
         size_t nn = (size_t)n * n;
-        double *d = nullptr;
-        HIPCHK(hipMalloc((void **)&d, sizeof(double) * 6 * nn));
+        DevBuf buf;
+        double *d = buf.alloc(sizeof(double) * 6 * nn);
         double *dA = d, *dB = d + nn, *dQh = d + 2 * nn, *dZ = d + 3 * nn, *dQ1 = d + 4 * nn, *dW = d + 5 * nn;
         HIPCHK(hipMemcpy2DAsync(dA, n * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpy2DAsync(dB, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice, L.st));
@@ -1182,7 +1199,6 @@ int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, i
             if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
         }
         HIPCHK(hipStreamSynchronize(L.st));
-        HIPCHK(hipFree(d));
         return info;
     } catch (const std::exception &e) {
         L.err = e.what();

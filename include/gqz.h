@@ -87,7 +87,9 @@ int gqz_geqrb_dev(char compq, int n, double *A, int lda, double *B, int ldb, dou
 
 /* ---- ScaLAPACK-descriptor surface for the Fortran shim (1x1 grid) ----------------------------
  * desc[9] = DTYPE, CTXT, M, N, MB, NB, RSRC, CSRC, LLD (PDHGEQZ.f:88-127).  Host pointers.
- * Workspace query when ldwork == -1 or liwork == -1 (PDHGEQZ.f:409-430). */
+ * gqz_pdhgeqz_host: workspace query when ldwork == -1 or liwork == -1, after the argument checks
+ * (PDHGEQZ.f:384-430).  gqz_pdgghrd_host: lwork == -1 returns MB in work[0] before any check
+ * (PDGGHRD.f:256-266); INFO -1..-7 as PDGGHRD.f:318-336. */
 int gqz_pdhgeqz_host(char job, char compq, char compz, int n, int ilo, int ihi,
                      double *H, const int *desch, double *T, const int *desct,
                      double *alphar, double *alphai, double *beta,

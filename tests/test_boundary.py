@@ -157,3 +157,51 @@ def test_bench_traffic_summary_is_consistent():
     want = 1024.0 * (e["fetch_kib"] * t["calibration"]["fetch_factor"] + e["write_kib"] * t["calibration"]["write_factor"])
     assert abs(bench.class_traffic("chase_window", 16000) - want) < 1e-6 * want
     assert bench.class_traffic("chase_window", 8000) is None       # other workloads: not covered
+
+
+def _gghrd_call(lib, cq=b"I", cz=b"I", n=8, ilo=1, ihi=None, nb=64, mb=None, lwork=None):
+    ihi = n if ihi is None else ihi
+    m = max(n, 1)
+    A = np.zeros((m, m), order="F"); B = np.eye(m, order="F")
+    Q = np.eye(m, order="F"); Z = np.eye(m, order="F")
+    lwork = (nb if mb is None else mb) if lwork is None else lwork
+    w = np.zeros(max(1, lwork, 1))
+    d = _desc(m, nb, mb)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.gqz_pdgghrd_host.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int]
+    info = lib.gqz_pdgghrd_host(cq, cz, n, ilo, ihi, vp(A), d, vp(B), d, vp(Q), d, vp(Z), d, vp(w), lwork)
+    return info, w
+
+
+@pytest.mark.parametrize("kw,expect", [
+    (dict(cq=b"X"), -1), (dict(cz=b"?"), -2), (dict(n=-1), -3), (dict(ilo=0), -4),
+    (dict(ihi=9), -5), (dict(ilo=5, ihi=3), -5), (dict(mb=32), -6), (dict(lwork=10), -7)])
+def test_pdgghrd_info_codes(lib, kw, expect):
+    """INFO codes of PDGGHRD.f:318-336 (checked before anything reaches a GPU)."""
+    info, _ = _gghrd_call(lib, **kw)
+    assert info == expect
+
+
+def test_pdgghrd_workspace_query_comes_first(lib):
+    """LWORK = -1 returns DESCA(MB_) in WORK(1) before any argument check (PDGGHRD.f:256-266)."""
+    info, w = _gghrd_call(lib, cq=b"X", nb=48, lwork=-1)
+    assert info == 0 and w[0] == 48
+
+
+def test_empty_ranges_are_accepted(lib):
+    """ILO = N + 1, IHI = N is a valid empty range for both routines (PDHGEQZ.f:392-395,
+    PDGGHRD.f:326-329): no INFO < 0; without a GPU the call then reports the missing device."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except Exception:
+        pass
+    info, _, _ = _host_call(lib, n=8, ilo=9, ihi=8)
+    assert info == -100
+    info, _ = _gghrd_call(lib, n=8, ilo=9, ihi=8)
+    assert info == -100
+    info, _ = _gghrd_call(lib, n=0, ilo=1, ihi=0)
+    assert info == 0
