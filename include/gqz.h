@@ -6,6 +6,8 @@
  * on the library's HIP stream; the host-pointer entry points copy in, run, and copy out.
  * Indices in this header are 1-based where they mirror a Fortran argument (ILO, IHI) and
  * 0-based nowhere else.  No torch types cross this boundary.
+ * Not re-entrant (like the reference, whose tolerances live in COMMON blocks): the stream, the
+ * workspace arena and the pinned staging buffers are shared, so concurrent callers serialise.
  */
 #ifndef GQZ_H
 #define GQZ_H
