@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256) k_gemm64(int ta, int tb, int m, int n, in
                                                 const double *__restrict__ A, int lda,
                                                 const double *__restrict__ B, int ldb,
                                                 double *__restrict__ C, int ldc) {
-    __shared__ double sA[64 * 17];   // 64 rows x 16 k  (ld 64 + padding by k-major 17?)
+    __shared__ double sA[64 * 17];   // k-major: 16 k-rows of 64 rows (entries 1024.. unused)
     __shared__ double sB[16 * 65];   // 16 k x 64 cols
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
