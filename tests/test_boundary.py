@@ -5,6 +5,8 @@ run (there is no CPU fallback).  No compute call reaches a GPU here."""
 import ctypes as C
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -205,3 +207,57 @@ def test_empty_ranges_are_accepted(lib):
     assert info == -100
     info, _ = _gghrd_call(lib, n=0, ilo=1, ihi=0)
     assert info == 0
+
+
+def _fortran_driver():
+    """Compile gevp-pdhgeqz_amd/fortran/exrand_gpu.f90, an EXRAND-style Fortran program, against
+    the shim and the library (ordinary Fortran linking, no ctypes)."""
+    fc = os.environ.get("GQZ_FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(FLIB) or not shutil.which(fc):
+        pytest.skip("Fortran shim or compiler missing")
+    obj = os.path.join(PKG, "_build")
+    os.makedirs(obj, exist_ok=True)
+    exe = os.path.join(obj, "exrand_gpu")
+    src = os.path.join(PKG, "fortran", "exrand_gpu.f90")
+    subprocess.run([fc, "-O2", "-J", obj, src, "-o", exe, "-L" + PKG, "-lpdhgeqz_f", "-lgqz",
+                    "-Wl,-rpath," + PKG], check=True, capture_output=True, timeout=300)
+    return exe
+
+
+def _key_values(text):
+    kv = {}
+    for line in text.splitlines():
+        f = line.split()
+        if len(f) >= 2 and f[0].replace("_", "").isalnum():
+            kv[f[0]] = f[1:]
+    return kv
+
+
+def test_fortran_program_argument_checks():
+    """A compiled Fortran caller sees the reference's INFO codes, workspace queries and PIPARMQ
+    values (PDHGEQZ.f:384-444, PDGGHRD.f:256-336, piparmq.f:440-517) through the shim."""
+    r = subprocess.run([_fortran_driver(), "args"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    kv = _key_values(r.stdout)
+    assert kv["pdhgeqz_bad_job"] == ["-1"]
+    assert kv["pdhgeqz_bad_ilo"] == ["-5"]
+    assert kv["pdhgeqz_small_nb"] == ["-7"]
+    assert kv["pdhgeqz_query"] == ["0", str(8 * 60), "16"]
+    assert kv["pdhgeqz_small_ldwork"] == ["-9"]
+    assert kv["pdgghrd_query"] == ["0", "60"]
+    assert kv["pdgghrd_bad_compq"] == ["-1"]
+    assert kv["pdgghrd_bad_ihi"] == ["-5"]
+    assert kv["pilaenvx_16000"] == ["512", "768", "5"]
+
+
+def test_fortran_program_without_gpu_stops():
+    """Without a GPU the Fortran program's first compute call returns INFO = -100 (no CPU path)."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except Exception:
+        pass
+    r = subprocess.run([_fortran_driver(), "run", "20"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert _key_values(r.stdout)["pdgghrd_info"] == ["-100"]
