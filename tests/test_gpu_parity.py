@@ -2,7 +2,8 @@
 (libgqz.so) and the Fortran shim, checked against the oracle on the same seeded inputs, against
 LAPACK (scipy) eigenvalues with condition-weighted chordal tolerances, and against the PCOMPRES
 residuals the reference's drivers gate on (testing/EXRAND1.f:329-332: RESA, RESB < 1e-14 there;
-1e-13 here, the bound the north-star tolerance names; RESQ, RESZ < 10 in units of n eps)."""
+1e-13 here, the backward-error bound this QZ is held to up to n = 16000; RESQ, RESZ < 10 in units
+of n eps)."""
 import ctypes as C
 import os
 
