@@ -153,7 +153,9 @@ class StaticPolicy:
 
 
 def run_traced(L, H0, T0, mult=None, learned=False, cap=100000):
-    """One QZ (JOB='S', COMPQ=COMPZ='I') of a copy of (H0, T0); returns (seconds, info, records)."""
+    """One QZ (JOB='S', COMPQ=COMPZ='I') of a copy of (H0, T0); returns (seconds, info, records,
+    residuals), the residuals being PCOMPRES's [resA, resB, resQ/(n eps), resZ/(n eps)] of the result
+    (computed after the timed call)."""
     import torch
     n = H0.shape[0]
     H = gqz.colmajor(n); T = gqz.colmajor(n); Q = gqz.colmajor(n); Z = gqz.colmajor(n)
@@ -171,13 +173,19 @@ def run_traced(L, H0, T0, mult=None, learned=False, cap=100000):
         _, _, _, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    res = gqz.compres(H0, T0, H, T, Q, Z)
     recs = (gqz.GqzTraceRec * cap)()
     k = L.gqz_trace_get(recs, cap)
     rows = [(np.array(recs[i].feat[:], dtype=np.float32), recs[i].nh, recs[i].ns, recs[i].nw, recs[i].nibble,
              recs[i].nd, recs[i].t_iter) for i in range(k)]
     L.gqz_trace_enable(0)
     del H, T, Q, Z
-    return dt, info, rows
+    return dt, info, rows, res
+
+
+def passes(res):
+    """The backward-error and orthogonality bar every QZ result is held to."""
+    return res[0] < 1e-13 and res[1] < 1e-13 and res[2] < 10.0 and res[3] < 10.0
 
 
 # ---- training ---------------------------------------------------------------------------------------------

@@ -8,8 +8,8 @@ relative to the PIPARMQ defaults, which the QZ evaluates once per call at the or
 block).  The first AED window W* of the fastest run is the target: every state recorded in the runs
 of that pencil is labelled log2(W* / NW_def(n)) for NW and 0 for NS and NIBBLE.  Larger pencils (--large-sizes, one family, a reduced grid) extend the
 range of active orders seen in training.  The perceptron is then fitted with
-the library's own FP32 gradient and Adam kernels.  Each run's backward error is checked: a run that
-fails it is never a target.
+the library's own FP32 gradient and Adam kernels.  Each run's backward error and orthogonality are
+checked (PCOMPRES on the device): a run that fails them, or returns INFO /= 0, is never a target.
 """
 import argparse
 import itertools
@@ -63,11 +63,14 @@ def main():
         H0, T0 = GP.make_pencil(fam, n, 1000 * seed + n)
         runs = []
         for mult in itertools.product(GRID_NS, grid_nw, GRID_NIB):
-            dt, info, rows = GP.run_traced(L, H0, T0, mult=mult)
-            runs.append((dt, info, mult, rows))
-            log("trace %s n=%d seed=%d mult=%s: %.2f s info=%d iterations=%d" % (fam, n, seed, mult, dt, info,
-                                                                                  len(rows)))
-        ok = [r for r in runs if r[1] == 0]
+            dt, info, rows, res = GP.run_traced(L, H0, T0, mult=mult)
+            runs.append((dt, info, mult, rows, res))
+            log("trace %s n=%d seed=%d mult=%s: %.2f s info=%d iterations=%d resA=%.1e resQ=%.2f"
+                % (fam, n, seed, mult, dt, info, len(rows), res[0], res[2]))
+        ok = [r for r in runs if r[1] == 0 and GP.passes(r[4])]
+        if not ok:
+            log("no run of %s n=%d seed=%d passed the checks: pencil skipped" % (fam, n, seed))
+            continue
         best = min(ok, key=lambda r: r[0])
         dflt = [r for r in ok if r[2] == (1.0, 1.0, 1.0)]
         wstar = best[3][0][3] if best[3] else GP.piparmq_nw(n)     # first AED window of the fastest run
