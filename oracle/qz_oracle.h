@@ -7,6 +7,27 @@
  *
  * Conventions: column-major FP64, 0-based indices, leading dimensions >= n.
  * Reference citations are relative to the NLAFET/GEVP-PDHGEQZ source tree.
+ *
+ * What pins each function (tests/test_oracle.py unless named; the reference ships no golden
+ * vectors, so the pins are closed forms, invariants, LAPACK via scipy and brute force):
+ *   orc_fill_*, orc_splitmix64    committed generator fixture (test_generator_fixture)
+ *   orc_lartg, orc_larfg          closed forms and annihilation (test_lartg_larfg)
+ *   orc_shift_col                 closed form of the shift polynomial's first column
+ *   orc_invhse3                   annihilation of T's first column (test_invhse_annihilates)
+ *   orc_lagv2                     standardised 2x2 forms, real and complex (test_lagv2_2x2_forms)
+ *   orc_eig22                     roots of the 2x2 characteristic quadratic
+ *   orc_tgex2, orc_tgexc          swapped spectra, structure and residuals
+ *   orc_geqrf_apply, orc_gghrd    HT structure and PCOMPRES residuals
+ *   orc_qz_ds                     Schur form, residuals and LAPACK eigenvalues
+ *   orc_hgeqz                     LAPACK eigenvalues (n = 1..260), the golden LAPACK fixture,
+ *                                 brute-force roots at n <= 6, exact triangular and B = I cases,
+ *                                 exactly known infinite counts
+ *   orc_chase_window              invariants of a chased chain (test_oracle_chase_window_equivalence)
+ *   orc_compres                   agreement with an independent numpy PCOMPRES
+ *   orc_piparmq_*                 the published PIPARMQ table (test_piparmq_table)
+ *   orc_mlp_forward, orc_mlp_grad closed form and central finite differences
+ * Parity unpinned (would need a run of the reference): the iteration trajectory of orc_hgeqz
+ * (per-iteration KTOP, KBOT, ND), the order of its eigenvalues and the particular S, T, Q, Z.
  */
 #ifndef QZ_ORACLE_H
 #define QZ_ORACLE_H

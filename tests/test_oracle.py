@@ -1,5 +1,6 @@
 """CPU tests: pin the oracle (oracle/qz_oracle.c) against LAPACK, closed forms, constructed
 pencils with known spectra and the committed golden fixtures.  No GPU needed."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -340,3 +341,77 @@ def test_oracle_standard_problem_b_identity():
     ratio, _, _ = Q.match_eigs(ar, ai, be, H, T)
     assert ratio < 1.0
     assert _max_matched_rel_err((ar + 1j * ai) / be, np.linalg.eigvals(H)) < 1e-8
+
+
+def test_compres_two_implementations_agree():
+    """The oracle's PCOMPRES (C) and the numpy restatement agree on a QZ result, and both see a
+    transposed Q as a backward-error failure."""
+    A, B = Q.pencil_randn(50, 31)
+    H, T, Qm, Zm = O.ht_reduce(A, B)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=2, compz=2, Q=Qm, Z=Zm)
+    c = O.compres(A, B, S, P, Q2, Z2)
+    p = np.array(Q.compres(A, B, S, P, Q2, Z2))
+    assert c[0] < 1e-14 and c[1] < 1e-14
+    # the residuals are themselves rounding errors: two summation orders agree to a few percent
+    np.testing.assert_allclose(c[2:], p[2:], rtol=0.05)
+    np.testing.assert_allclose(c[:2], p[:2], rtol=0.2, atol=2e-16)
+    bad = O.compres(A, B, S, P, Q2.T.copy(), Z2)
+    assert bad[0] > 1e-3 and Q.compres(A, B, S, P, Q2.T.copy(), Z2)[0] > 1e-3
+
+
+def test_base_double_shift_qz():
+    """orc_qz_ds (the small-block QZ) on its own: Schur form of an HT pencil with the PCOMPRES
+    bounds and LAPACK's eigenvalues."""
+    n = 30
+    H0, T0 = Q.pencil_ht_random(n, 5)
+    H, T = O.fmat(H0.copy()), O.fmat(T0.copy())
+    Qm, Zm = O.fmat(np.eye(n)), O.fmat(np.eye(n))
+    ar, ai, be = np.zeros(n), np.zeros(n), np.zeros(n)
+    ttol = np.finfo(float).eps * np.linalg.norm(T0)
+    info = O.lib().orc_qz_ds(n, 0, n - 1, O.ptr(H), n, O.ptr(T), n, O.ptr(ar), O.ptr(ai), O.ptr(be),
+                             O.ptr(Qm), n, O.ptr(Zm), n, ttol, None)
+    assert info == 0
+    ra, rb, rq, rz = Q.compres(H0, T0, H, T, Qm, Zm)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+    below, tb, ok2 = Q.schur_structure(H, T)
+    assert below == 0.0 and tb == 0.0 and ok2
+    ratio, _, _ = Q.match_eigs(ar, ai, be, H0, T0)
+    assert ratio < 1.0
+
+
+def test_tgexc_moves_a_block_to_the_top():
+    """DTGEXC semantics (src/reorder/dtgexc.f): a real pencil with real eigenvalues in Schur form;
+    moving the last 1x1 block to the top puts its eigenvalue first and shifts the others down by one,
+    with Q, Z updated so that the backward error stays at rounding level."""
+    n = 12
+    A, B, lam, _ = Q.pencil_known(n, 17)
+    H, T, Qm, Zm = O.ht_reduce(A, B)
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=2, compz=2, Q=Qm, Z=Zm)
+    assert info == 0 and np.all(ai == 0.0)
+    before = np.diag(S) / np.diag(P)
+    ifst, ilst = C.c_int(n - 1), C.c_int(0)
+    S, P, Q2, Z2 = (O.fmat(x) for x in (S, P, Q2, Z2))
+    r = O.lib().orc_tgexc(n, O.ptr(S), n, O.ptr(P), n, O.ptr(Q2), n, O.ptr(Z2), n, C.byref(ifst), C.byref(ilst))
+    assert r == 0
+    after = np.diag(S) / np.diag(P)
+    np.testing.assert_allclose(after, np.concatenate([[before[-1]], before[:-1]]), rtol=1e-10)
+    assert np.abs(np.tril(S, -1)).max() < 1e-13 * np.abs(S).max() and np.abs(np.tril(P, -1)).max() == 0.0
+    ra, rb, rq, rz = Q.compres(A, B, S, P, Q2, Z2)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+
+
+@pytest.mark.parametrize("A,B", [
+    ([[1.0, 2.0], [3.0, 4.0]], [[2.0, 1.0], [0.0, 1.0]]),        # real pair
+    ([[1.0, -5.0], [2.0, 1.0]], [[1.0, 0.5], [0.0, 2.0]]),       # complex pair
+    ([[0.0, 1.0], [-1.0, 0.0]], [[1.0, 0.0], [0.0, 1.0]]),       # +-i
+])
+def test_eig22_quadratic_roots(A, B):
+    """Eigenvalues of a 2x2 pencil with B upper triangular are the roots of
+    det(A - l B) = b00 b11 l^2 - (a00 b11 + a11 b00 - a10 b01) l + (a00 a11 - a01 a10)."""
+    A = np.array(A); B = np.array(B)
+    ar, ai, b = np.zeros(2), np.zeros(2), np.zeros(2)
+    O.lib().orc_eig22(O.ptr(O.fmat(A)), 2, O.ptr(O.fmat(B)), 2, O.ptr(ar), O.ptr(ai), O.ptr(b))
+    coef = [B[0, 0] * B[1, 1], -(A[0, 0] * B[1, 1] + A[1, 1] * B[0, 0] - A[1, 0] * B[0, 1]),
+            A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]]
+    ref = np.roots(coef)
+    assert _max_matched_rel_err((ar + 1j * ai) / b, ref) < 1e-13
