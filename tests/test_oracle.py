@@ -415,3 +415,37 @@ def test_eig22_quadratic_roots(A, B):
             A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]]
     ref = np.roots(coef)
     assert _max_matched_rel_err((ar + 1j * ai) / b, ref) < 1e-13
+
+
+def test_oracle_job_and_comp_variants():
+    """JOB = 'E' and COMPQ/COMPZ = 'N' give the same spectrum as the full Schur form (PDHGEQZ.f:132-150);
+    the factors that are computed stay orthogonal."""
+    n = 90
+    H, T = Q.pencil_ht_random(n, 13)
+    for job, cq, cz in ((0, 0, 0), (1, 0, 1), (1, 1, 0), (1, 1, 1)):
+        S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, job=job, compq=cq, compz=cz)
+        assert info == 0
+        ratio, _, _ = Q.match_eigs(ar, ai, be, H, T)
+        assert ratio < 1.0
+        for on, M in ((cq, Q2), (cz, Z2)):
+            if on:
+                assert np.linalg.norm(M.T @ M - np.eye(n)) / (n * Q.EPS) < 10
+
+
+def test_oracle_ilo_ihi_subblock():
+    """An active block [ILO, IHI] of a pencil that is already triangular outside it: Q and Z only
+    touch the block's rows and columns, the residuals hold, and the block's eigenvalues are LAPACK's."""
+    n, ilo, ihi = 80, 7, 65            # 0-based bounds of the block
+    H, T = Q.pencil_ht_random(n, 14)
+    H[ilo, ilo - 1] = 0.0
+    H[ihi + 1, ihi] = 0.0
+    S, P, ar, ai, be, Q2, Z2, info, st = O.hgeqz(H, T, compq=1, compz=1, ilo=ilo, ihi=ihi)
+    assert info == 0
+    ra, rb, rq, rz = Q.compres(H, T, S, P, Q2, Z2)
+    assert ra < 1e-14 and rb < 1e-14 and rq < 10 and rz < 10
+    I = np.eye(n)
+    assert np.array_equal(Q2[:ilo], I[:ilo]) and np.array_equal(Q2[ihi + 1:], I[ihi + 1:])
+    assert np.array_equal(Z2[:, :ilo], I[:, :ilo]) and np.array_equal(Z2[:, ihi + 1:], I[:, ihi + 1:])
+    b = slice(ilo, ihi + 1)
+    ratio, _, _ = Q.match_eigs(ar[b], ai[b], be[b], H[b, b], T[b, b])
+    assert ratio < 1.0
