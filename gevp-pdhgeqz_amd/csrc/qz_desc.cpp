@@ -31,42 +31,12 @@ int decode(char c, const char *opts) {
 bool desc_1x1_ok(const int *d, int n) {
     return d && d[RSRC_] == 0 && d[CSRC_] == 0 && d[LLD_] >= std::max(1, n);
 }
-int piparmq_ns(int nh) {
-    int ns = 2;
-    if (nh >= 30) ns = 4;
-    if (nh >= 60) ns = 10;
-    if (nh >= 150) {
-        int lg = (int)std::lround(std::log((double)nh) / std::log(2.0));
-        ns = std::max(10, nh / lg);
-    }
-    if (nh >= 590) ns = 64;
-    if (nh >= 3000) ns = 128;
-    if (nh >= 6000) ns = 256;
-    if (nh >= 12000) ns = 512;
-    if (nh >= 24000) ns = 1024;
-    if (nh >= 48000) ns = 2048;
-    return std::max(2, ns - ns % 2);
-}
 }  // namespace
 
 extern "C" {
 
 int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, int lda, double *B, int ldb,
                    double *Q, int ldq, double *Z, int ldz);
-
-// PIPARMQ / PILAENVX values used by the Fortran shim (piparmq.f:300-531): ISPEC 50 NMIN1,
-// 51 NW, 52 NIBBLE (PDHGEQZ0 form), 53 NS, 54 NMIN2, 55 NMIN3.
-int gqz_piparmq(int ispec, int nh) {
-    switch (ispec) {
-    case 50: return 6000;
-    case 51: { int ns = piparmq_ns(nh); return nh <= 500 ? ns : 3 * ns / 2; }
-    case 52: return nh < 6000 ? 14 : std::min(100, (int)std::ceil(335.0 * std::pow((double)nh, -0.44)));
-    case 53: return piparmq_ns(nh);
-    case 54: return 201;
-    case 55: return 201;
-    default: return -1;
-    }
-}
 
 int gqz_pdhgeqz_host(char job, char compq, char compz, int n, int ilo, int ihi, double *H, const int *desch,
                      double *T, const int *desct, double *alphar, double *alphai, double *beta, double *Q,

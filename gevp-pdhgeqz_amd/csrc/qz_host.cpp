@@ -745,6 +745,20 @@ void gqz_set_error(const char *msg) { L.err = msg; }
 extern "C" {
 
 const char *gqz_version(void) { return "gqz 0.1 (gfx950, fp64)"; }
+
+// PIPARMQ / PILAENVX values (piparmq.f:300-531), the same functions the QZ uses for its defaults:
+// ISPEC 50 NMIN1, 51 NW, 52 NIBBLE (PDHGEQZ0 form), 53 NS, 54 NMIN2, 55 NMIN3.
+int gqz_piparmq(int ispec, int nh) {
+    switch (ispec) {
+    case 50: return 6000;
+    case 51: return piparmq_nw(nh);
+    case 52: return piparmq_nibble(nh);
+    case 53: return piparmq_ns(nh);
+    case 54: return 201;
+    case 55: return 201;
+    default: return -1;
+    }
+}
 const char *gqz_last_error(void) { return L.err.c_str(); }
 void *gqz_stream(void) { return (void *)L.st; }
 
