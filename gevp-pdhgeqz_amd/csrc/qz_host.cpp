@@ -787,18 +787,31 @@ int gqz_init(int device) {
     }
 }
 
+// Releases what gqz_init and the calls since allocated.  Every release is attempted even after a
+// failure; the first failure is reported (-1, text in gqz_last_error).
 int gqz_finalize(void) {
     if (L.dev < 0) return 0;
-    hipSetDevice(L.dev);
-    hipStreamSynchronize(L.st);
-    if (L.ws.base) hipFree(L.ws.base);
+    hipError_t first = hipSuccess;
+    auto note = [&first](hipError_t e) { if (first == hipSuccess) first = e; };
+    note(hipSetDevice(L.dev));
+    note(hipStreamSynchronize(L.st));
+    if (L.ws.base) note(hipFree(L.ws.base));
     L.ws = Arena();
-    hipFree(L.d_out); hipFree(L.d_norm); hipHostFree(L.h_out); hipHostFree(L.h_buf);
-    if (L.d_mlp) { hipFree(L.d_mlp); hipFree(L.d_mlp_x); hipFree(L.d_mlp_y); hipHostFree(L.h_mlp_io); }
+    note(hipFree(L.d_out)); note(hipFree(L.d_norm));
+    note(hipHostFree(L.h_out)); note(hipHostFree(L.h_buf));
+    if (L.d_mlp) {
+        note(hipFree(L.d_mlp)); note(hipFree(L.d_mlp_x)); note(hipFree(L.d_mlp_y));
+        note(hipHostFree(L.h_mlp_io));
+    }
+    L.d_out = nullptr; L.d_norm = nullptr; L.h_out = nullptr; L.h_buf = nullptr;
     L.d_mlp = L.d_mlp_x = L.d_mlp_y = L.h_mlp_io = nullptr;
     L.mlp_on = false;
-    hipStreamDestroy(L.st);
+    note(hipStreamDestroy(L.st));
     L.st = nullptr; L.dev = -1;
+    if (first != hipSuccess) {
+        L.err = std::string("gqz_finalize: ") + hipGetErrorString(first);
+        return -1;
+    }
     return 0;
 }
 
@@ -809,7 +822,7 @@ int gqz_sync(void) {
 
 int gqz_ktime_enable(unsigned mask) {
     if (L.dev >= 0) {
-        hipStreamSynchronize(L.st);
+        if (hipStreamSynchronize(L.st) != hipSuccess) { L.err = "gqz_ktime_enable: stream error"; return -1; }
         kt_drain();
     }
     KT.mask = mask;

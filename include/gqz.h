@@ -49,7 +49,7 @@ typedef void (*gqz_policy_fn)(void *user, const float feat[12], int nh, int *ns,
 
 /* ---- library ---------------------------------------------------------------------------- */
 int gqz_init(int device);                 /* select the device, create stream + workspace    */
-int gqz_finalize(void);
+int gqz_finalize(void);                   /* releases the device state; -1 after a HIP error  */
 const char *gqz_version(void);
 void *gqz_stream(void);                   /* hipStream_t used by the _dev entry points        */
 int gqz_sync(void);
