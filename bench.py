@@ -1,26 +1,37 @@
 """Benchmark of the multishift QZ hot path on one MI355X.
 
-Workload (default): Schur form (JOB='S', COMPQ=COMPZ='I') of an n=16000 Hessenberg-triangular pencil
-drawn as in testing/EXRAND1.f:243-279 (N(0,1) upper parts, chi-distributed subdiagonal of H and
-diagonal of T: the distribution of an HT-reduced N(0,1) pencil), with the learned parameter policy
-(default) or the PIPARMQ default parameters (--params default: NS=512, NW=768, NIBBLE=5 at n=16000).
-A "step" is one complete QZ of that pencil; the pencil is generated on the device and restored from a
-pristine device copy before each step (inputs resident in HBM when timing starts).  After the timed
-steps the result is checked (PCOMPRES residuals) and, with --compare-n m, one pencil of order m is
-solved with both parameter sets for the learned-versus-default comparison.
+Workload: Schur form (JOB='S', COMPQ=COMPZ='I') of an n=16000 Hessenberg-triangular pencil drawn as in
+testing/EXRAND1.f:243-279 (N(0,1) upper parts, chi-distributed subdiagonal of H and diagonal of T: the
+distribution of an HT-reduced N(0,1) pencil), with the learned parameter policy.  A "step" is one
+complete QZ of that pencil; the pencil is generated on the device and restored from a pristine device
+copy before each step (inputs resident in HBM when timing starts).
 
-Output: one JSON line with the QZ time in seconds, the roofline of the dominant kernel class and of
+The run keeps to a wall-clock budget measured from process start (--budget-s, default 470 s):
+  1. warmup QZ calls on a small pencil (at most --warmup, at most 20 s),
+  2. timed learned-parameter steps at n (at least one; --steps is a maximum, further steps only while
+     they fit the share of the budget given to them),
+  3. the PCOMPRES check of the last step, on the device,
+  4. one batched pass (4096 pencils, n log-uniform in [32, 1024], pencils/s) as a sub-field,
+  5. the CPU baseline: the oracle's QZ on a bounded sample (n = 1024, one core), scaled by n^3,
+  6. the same n pencil with the PIPARMQ default parameters (NS=512, NW=768, NIBBLE=5 at n=16000) in a
+     worker thread; when it cannot finish inside the budget the line reports the elapsed time as a
+     lower bound of the default time instead.
+Exactly one JSON line is printed on stdout, before the budget ends, whatever finished.
+
+Output fields: metric/value (learned QZ seconds at n), roofline of the dominant kernel class and of
 the MFMA slab update (HIP-event time on the library stream, algorithmic bytes or flops of their
-launches), the comparison, and a CPU baseline (the oracle's QZ on a bounded sample, cubically
-extrapolated to the workload's n).
---workload batched measures the batched mode instead (pencils per second).
+launches), learned_vs_default, batched, cpu_baseline, check.
+--workload batched measures only the batched mode (pencils per second).
 """
 import argparse
 import ctypes as C
 import json
 import os
 import sys
+import threading
 import time
+
+T_START = time.perf_counter()
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "gevp-pdhgeqz_amd"), os.path.join(ROOT, "tests"), ROOT):
@@ -33,6 +44,14 @@ RIDGE = PEAK_FP64 / PEAK_HBM
 # hardware byte counts per dispatch of the QZ kernels on the default workload (two rocprofv3 --pmc
 # passes, FETCH_SIZE and WRITE_SIZE, scaled by the same counters on kernels of known size)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "qz_n16000_learned_traffic.json")
+
+
+def elapsed():
+    return time.perf_counter() - T_START
+
+
+def log(*a):
+    print("[%6.1f s]" % elapsed(), *a, file=sys.stderr, flush=True)
 
 
 def class_traffic(name, n):
@@ -87,19 +106,17 @@ def ht_random_device(n, seed, torch, gqz):
     return H, T
 
 
-def batched_workload(args, torch, gqz, L, log):
-    """4096 independent N(0,1) pencils with n = round(32 * 32^u), u ~ U(0,1) (log-uniform in
+def batched_workload(count, seed, steps, warmup, tune, torch, gqz, L):
+    """count independent N(0,1) pencils with n = round(32 * 32^u), u ~ U(0,1) (log-uniform in
     [32, 1024]): QR(B) + HT reduction + QZ with Q and Z, one persistent launch replayed as a hipGraph,
     launch shape from the autotuner.  A step is one pass over the whole batch."""
-    import ctypes as C
     import numpy as np
-    rng = np.random.default_rng(args.seed)
-    count = args.batch
+    rng = np.random.default_rng(seed)
     ns = np.round(32.0 * 32.0 ** rng.uniform(0, 1, count)).astype(np.int32)
     off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]]).astype(np.int64)
     eoff = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]]).astype(np.int64)
     tot = int((ns.astype(np.int64) ** 2).sum()); et = int(ns.sum())
-    g = torch.Generator(device="cuda"); g.manual_seed(args.seed)
+    g = torch.Generator(device="cuda"); g.manual_seed(seed)
     A0 = torch.randn(tot, dtype=torch.float64, device="cuda", generator=g)
     B0 = torch.randn(tot, dtype=torch.float64, device="cuda", generator=g)
     A = torch.empty_like(A0); Bm = torch.empty_like(B0); Q = torch.empty_like(A0); Z = torch.empty_like(A0)
@@ -107,21 +124,20 @@ def batched_workload(args, torch, gqz, L, log):
     info = torch.zeros(count, dtype=torch.int32, device="cuda")
     vp = lambda t: C.c_void_p(t.data_ptr())
     bt = C.c_int(0); bw = C.c_int(0); bms = C.c_double(0)
-    if args.tune:
-        log("autotuner on a sample of the size distribution ...")
+    if tune:
         t0 = time.perf_counter()
         L.gqz_batched_tune(count, ns.ctypes.data_as(C.c_void_p), C.byref(bt), C.byref(bw), C.byref(bms))
-        log("autotuner: %d threads, %d workgroups (%.1f ms on the sample) in %.1f s"
+        log("batched autotuner: %d threads, %d workgroups (%.1f ms on the sample) in %.1f s"
             % (bt.value, bw.value, bms.value, time.perf_counter() - t0))
     args_c = lambda: (count, ns.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
                       eoff.ctypes.data_as(C.c_void_p), vp(A), vp(Bm), vp(Q), vp(Z), vp(ar), vp(ai), vp(be),
                       vp(info), 0)
-    for w in range(args.warmup):
+    for w in range(warmup):
         A.copy_(A0); Bm.copy_(B0)
         torch.cuda.synchronize()
         assert L.gqz_batched_dev(*args_c()) == 0
     times = []
-    for s in range(args.steps):
+    for s in range(steps):
         A.copy_(A0); Bm.copy_(B0)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -139,15 +155,18 @@ def batched_workload(args, torch, gqz, L, log):
         res = gqz.compres(M(A0), M(B0), M(A), M(Bm), M(Q), M(Z))
         worst = [max(a, b) for a, b in zip(worst, res)]
     flops = float((66.0 * ns.astype(np.float64) ** 3).sum())
-    return times, {"count": count, "n_min": int(ns.min()), "n_max": int(ns.max()), "sum_n3": float((ns.astype(np.float64) ** 3).sum()),
-                   "threads": bt.value, "info_nonzero": int((info != 0).sum().item()),
-                   "worst_resA": worst[0], "worst_resB": worst[1], "worst_resQ_neps": worst[2],
-                   "worst_resZ_neps": worst[3], "gflops_66n3": flops / min(times) / 1e9}
+    out = {"count": count, "n_min": int(ns.min()), "n_max": int(ns.max()),
+           "sum_n3": float((ns.astype(np.float64) ** 3).sum()), "threads": bt.value,
+           "info_nonzero": int((info != 0).sum().item()), "worst_resA": worst[0], "worst_resB": worst[1],
+           "worst_resQ_neps": worst[2], "worst_resZ_neps": worst[3],
+           "gflops_66n3": flops / min(times) / 1e9, "step_seconds": times,
+           "pencils_per_s": count / min(times)}
+    del A0, B0, A, Bm, Q, Z
+    return times, out
 
 
 def cpu_baseline(n_target, n_sample, seed):
-    """Oracle QZ (the C restatement, one core) on an n_sample HT-random pencil; cubic extrapolation."""
-    import numpy as np
+    """Oracle QZ (the C restatement, one core) on an n_sample HT-random pencil; cubic scaling."""
     import oracle_ffi as O
     import qzcheck as QC
     H, T = QC.pencil_ht_random(n_sample, seed)
@@ -155,20 +174,46 @@ def cpu_baseline(n_target, n_sample, seed):
     _, _, _, _, _, _, _, info, _ = O.hgeqz(H, T)
     dt = time.perf_counter() - t0
     return {"value": dt * (n_target / n_sample) ** 3, "unit": "s", "cores": 1, "kind": "port",
-            "sample": "oracle QZ (JOB=S, COMPQ=COMPZ=I) of one n=%d HT-random pencil: %.2f s measured on 1 core, "
-                      "scaled by (n/%d)^3 to n=%d; info=%d" % (n_sample, dt, n_sample, n_target, info)}
+            "sample": "oracle QZ (JOB=S, COMPQ=COMPZ=I) of one n=%d HT-random pencil of the same family: "
+                      "%.2f s measured on 1 core, scaled by (n/%d)^3 to n=%d; info=%d"
+                      % (n_sample, dt, n_sample, n_target, info)}
+
+
+class Emitter:
+    """Holds the fields of the one JSON line and prints it exactly once (at the end of the run, or
+    from the watchdog when the budget runs out)."""
+
+    def __init__(self):
+        self.out = None
+        self.lock = threading.Lock()
+        self.done = False
+
+    def emit(self, extra_note=None):
+        with self.lock:
+            if self.done or self.out is None:
+                return False
+            if extra_note:
+                self.out.setdefault("notes", []).append(extra_note)
+            print(json.dumps(self.out), flush=True)
+            self.done = True
+            return True
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="maximum number of timed steps")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--params", choices=["default", "learned"], default="learned")
-    ap.add_argument("--compare-n", type=int, default=4000,
-                    help="order of an extra default-vs-learned comparison run (0: none)")
+    ap.add_argument("--budget-s", type=float, default=470.0,
+                    help="wall-clock budget from process start; the JSON line is printed before it ends")
+    ap.add_argument("--step-share", type=float, default=0.15,
+                    help="share of the budget the timed steps may use beyond the first one")
+    ap.add_argument("--compare", type=int, default=1,
+                    help="learned versus default parameters on the same pencil (n) after the timed steps")
+    ap.add_argument("--batched", type=int, default=4096, help="pencils of the batched pass (0: none)")
     ap.add_argument("--warmup-n", type=int, default=2000, help="pencil order of the warmup steps")
     ap.add_argument("--cpu-sample-n", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -177,7 +222,6 @@ def main():
     ap.add_argument("--ktime-every", type=int, default=16,
                     help="time one launch in k (pseudo-random) of each kernel class (events cost queue packets)")
     ap.add_argument("--workload", choices=["qz", "batched"], default="qz")
-    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tune", type=int, default=1, help="run the batched autotuner first")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first fill, then sum, a device buffer of this many MiB (kernels of known byte "
@@ -187,29 +231,26 @@ def main():
         print("bench.py runs on one GPU only: this QZ build has no multi-GPU path (see the sharding "
               "design); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    budget = args.budget_s
 
     import torch
     import gqz
     L = gqz.init(0)
+    log("library loaded: %s" % L.gqz_version().decode())
     L.gqz_ktime_enable.argtypes = [C.c_uint]
     L.gqz_ktime_get.argtypes = [C.POINTER(KTime), C.c_int]
     L.gqz_ktime_sample.argtypes = [C.c_int]
     L.gqz_ktime_sample(args.ktime_every)
+    L.gqz_progress.argtypes = [C.c_double]
+    L.gqz_progress(30.0)       # long QZ calls report their progress on stderr
     n = args.n
-    prm = gqz.GqzParams()
-    if args.params == "learned":
-        import gqz_policy
-        gqz_policy.install_learned(L)
-        prm = gqz_policy.learned_params()
-
-    def log(*a):
-        print(*a, file=sys.stderr, flush=True)
 
     if args.workload == "batched":
-        times, info = batched_workload(args, torch, gqz, L, log)
+        times, info = batched_workload(args.batched or 4096, args.seed, max(1, args.steps), args.warmup, args.tune,
+                                       torch, gqz, L)
         out = {"metric": "batched pencils/s (QR + HT + QZ with Q, Z; %d pencils, n log-uniform in [32, 1024])"
                          % info["count"],
-               "value": info["count"] / min(times), "unit": "pencils/s", "n_gpus": 1, "steps": args.steps,
+               "value": info["count"] / min(times), "unit": "pencils/s", "n_gpus": 1, "steps": len(times),
                "warmup": args.warmup, "ms_per_step": 1000.0 * sum(times) / len(times), "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                "data": "synthetic: A, B i.i.d. N(0,1) generated on the device, seed %d" % args.seed,
@@ -217,6 +258,12 @@ def main():
                "roofline": None, "cpu_baseline": None}
         print(json.dumps(out), flush=True)
         return
+
+    import gqz_policy
+    gqz_policy.install_learned(L)
+    prm_learned = gqz_policy.learned_params()
+    prm_default = gqz.GqzParams()
+    prm = prm_learned if args.params == "learned" else prm_default
 
     if args.calib_mb > 0:
         cb = torch.empty(args.calib_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
@@ -226,7 +273,10 @@ def main():
         log("calibration: filled and summed %d MiB (sum %.0f)" % (args.calib_mb, tot))
         del cb
     # warmup steps (same pipeline, smaller pencil: loads code objects, sizes the workspace)
+    t_w = time.perf_counter()
     for w in range(args.warmup):
+        if time.perf_counter() - t_w > 20.0:
+            break
         Hw, Tw = ht_random_device(args.warmup_n, 1000 + w, torch, gqz)
         Qw = gqz.colmajor(args.warmup_n); Zw = gqz.colmajor(args.warmup_n)
         _, _, _, info, _ = gqz.pdhgeqz("S", "I", "I", Hw, Tw, Qw, Zw, params=prm)
@@ -239,10 +289,16 @@ def main():
     torch.cuda.synchronize()
     L.gqz_ktime_enable((1 << 7) - 1 if args.ktime else 0)
     times, stats, infos = [], [], []
+    step_budget = args.step_share * budget
     torch.cuda.synchronize()
     t_all0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(max(1, args.steps)):
+        if s > 0:
+            # another step only when it fits the step share and the rest of the budget
+            if (time.perf_counter() - t_all0) + times[-1] > step_budget or elapsed() + 2 * times[-1] > budget:
+                break
         H.copy_(H0); T.copy_(T0)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
         torch.cuda.synchronize()
@@ -250,13 +306,14 @@ def main():
         stats.append(st); infos.append(info)
         log("step %d: n=%d %.3f s info=%d sweeps=%d aed=%d" % (s, n, times[-1], info, st["sweeps"], st["aed_calls"]))
     torch.cuda.synchronize()
-    t_all = time.perf_counter() - t_all0
+    t_steps = sum(times)
     kt = (KTime * 16)()
     nk = L.gqz_ktime_get(kt, 16)
     L.gqz_ktime_enable(0)
     classes = [dict(name=kt[i].name.decode(), launches=kt[i].launches, ms=kt[i].ms, flops=kt[i].flops,
                     bytes=kt[i].bytes) for i in range(max(nk, 0))]
-    ms_per_step = 1000.0 * t_all / args.steps
+    nsteps = len(times)
+
     def roofline(c):
         avg_s = c["ms"] / 1000.0 / c["launches"]
         if c["flops"] > 0 and c["flops"] / max(c["bytes"], 1.0) > RIDGE:
@@ -269,8 +326,8 @@ def main():
         r["traffic"] = class_traffic(c["name"], n) if args.params == "learned" else None
         r["kernel"] = c["name"]
         r["avg_launch_us"] = 1e6 * avg_s
-        r["launches_per_step"] = c["launches"] / args.steps
-        r["share_of_step"] = c["ms"] / 1000.0 / t_all
+        r["launches_per_step"] = c["launches"] / nsteps
+        r["share_of_step"] = c["ms"] / 1000.0 / t_steps
         if c["flops"] > 0:
             r["tflops"] = c["flops"] / c["launches"] / avg_s / 1e12
             r["gbytes_per_s"] = c["bytes"] / c["launches"] / avg_s / 1e9
@@ -282,50 +339,17 @@ def main():
         slabs = [c for c in live if c["name"] == "slab_update"]
         if slabs:
             roof_slab = roofline(slabs[0])
-    check = None
-    if not args.no_check:
-        res = gqz.compres(H0, T0, H, T, Q, Z)
-        be_h = be.cpu()
-        check = {"resA": res[0], "resB": res[1], "resQ_neps": res[2], "resZ_neps": res[3],
-                 "n_infinite": int((be_h == 0).sum()), "info": infos[-1]}
-        log("check:", check)
-    comparison = None
-    if args.compare_n > 0:
-        # learned versus default parameters on a second pencil of the same family (not timed in value)
-        import gqz_policy
-        m = args.compare_n
-        Hc0, Tc0 = ht_random_device(m, args.seed + 1, torch, gqz)
-        comparison = {"n": m}
-        for mode in ("default", "learned"):
-            if mode == "learned":
-                gqz_policy.install_learned(L)
-                pc = gqz_policy.learned_params()
-            else:
-                pc = gqz.GqzParams()
-            Hc = gqz.colmajor(m); Tc = gqz.colmajor(m); Qc = gqz.colmajor(m); Zc = gqz.colmajor(m)
-            Hc.copy_(Hc0); Tc.copy_(Tc0)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _, _, _, infoc, _ = gqz.pdhgeqz("S", "I", "I", Hc, Tc, Qc, Zc, params=pc)
-            torch.cuda.synchronize()
-            comparison[mode + "_s"] = time.perf_counter() - t0
-            rc = gqz.compres(Hc0, Tc0, Hc, Tc, Qc, Zc)
-            comparison[mode + "_resA"] = rc[0]
-            log("comparison n=%d %s: %.3f s info=%d resA=%.2e" % (m, mode, comparison[mode + "_s"], infoc, rc[0]))
-            del Hc, Tc, Qc, Zc
-        comparison["gain"] = comparison["default_s"] / comparison["learned_s"]
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, args.cpu_sample_n, args.seed)
     st = stats[-1]
-    out = {
-        "metric": "QZ seconds, n=%d pencil, %s parameters (bkwd err checked)" % (n, args.params),
-        "value": t_all / args.steps,
+    em = Emitter()
+    em.out = {
+        "metric": "QZ seconds, n=%d pencil, %s parameters (bkwd err checked); learned vs default; batched pencils/s"
+                  % (n, args.params),
+        "value": t_steps / nsteps,
         "unit": "s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": nsteps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1000.0 * t_steps / nsteps,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
@@ -333,17 +357,95 @@ def main():
         "data": "synthetic: EXRAND1-style random Hessenberg-triangular pencil generated on the device, seed %d"
                 % args.seed,
         "config": {"workload": "multishift QZ (JOB=S, COMPQ=COMPZ=I) of an n=%d HT pencil" % n, "n": n,
-                   "params": args.params, "ns": st.get("shifts_used", 0), "sweeps": st["sweeps"],
-                   "aed_calls": st["aed_calls"], "iterations": st["iters"]},
+                   "params": args.params, "shifts_used": st.get("shifts_used", 0), "sweeps": st["sweeps"],
+                   "aed_calls": st["aed_calls"], "iterations": st["iters"], "swap_fail": st["swap_fail"],
+                   "steps_requested": args.steps, "budget_s": budget},
         "roofline": roof,
         "roofline_slab_update": roof_slab,
-        "cpu_baseline": cpu,
-        "learned_vs_default": comparison,
-        "check": check,
+        "cpu_baseline": None,
+        "learned_vs_default": None,
+        "batched": None,
+        "check": None,
         "kernel_classes": classes,
         "step_seconds": times,
     }
-    print(json.dumps(out), flush=True)
+
+    # watchdog: print whatever exists when the budget runs out and end the process (a run still on
+    # the GPU is abandoned; nothing else is pending)
+    def watchdog():
+        while not em.done:
+            if elapsed() > budget:
+                if em.emit("budget of %.0f s reached at %.1f s: remaining parts not measured" % (budget, elapsed())):
+                    os._exit(0)
+                return
+            time.sleep(0.5)
+    threading.Thread(target=watchdog, daemon=True).start()
+
+    if not args.no_check:
+        res = gqz.compres(H0, T0, H, T, Q, Z)
+        be_h = be.cpu()
+        em.out["check"] = {"resA": res[0], "resB": res[1], "resQ_neps": res[2], "resZ_neps": res[3],
+                           "n_infinite": int((be_h == 0).sum()), "info": infos[-1],
+                           "passes": bool(res[0] < 1e-13 and res[1] < 1e-13 and res[2] < 10 and res[3] < 10)}
+        log("check:", em.out["check"])
+    torch.cuda.empty_cache()
+
+    if args.batched > 0 and elapsed() + 90 < budget:
+        try:
+            btimes, binfo = batched_workload(args.batched, args.seed, 1, 0, args.tune, torch, gqz, L)
+            em.out["batched"] = binfo
+        except Exception as e:    # reported, the QZ line still stands
+            em.out["batched"] = {"error": str(e)}
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu_baseline and elapsed() + 40 < budget:
+        em.out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample_n, args.seed)
+        log("cpu baseline:", em.out["cpu_baseline"]["sample"])
+
+    if args.compare and args.params == "learned":
+        # the same pencil with the PIPARMQ defaults, in a worker thread (ctypes releases the GIL); the
+        # main thread waits until the budget and otherwise reports a lower bound
+        learned_s = em.out["value"]
+        cmp_ = {"n": n, "learned_s": learned_s, "default_s": None}
+        em.out["learned_vs_default"] = cmp_
+        res_box = {}
+
+        def run_default():
+            H.copy_(H0); T.copy_(T0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, bed, infod, std = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm_default)
+            torch.cuda.synchronize()
+            res_box["t"] = time.perf_counter() - t0
+            res_box["info"] = infod
+            res_box["stats"] = std
+        log("default-parameter run at n=%d (PIPARMQ: NS=%d, NW=%d), %.0f s of budget left"
+            % (n, gqz_policy.piparmq_ns(n), gqz_policy.piparmq_nw(n), budget - elapsed()))
+        t_d0 = time.perf_counter()
+        th = threading.Thread(target=run_default, daemon=True)
+        th.start()
+        while th.is_alive() and elapsed() < budget - 2.0:
+            th.join(0.5)
+        if "t" in res_box:
+            cmp_["default_s"] = res_box["t"]
+            cmp_["default_info"] = res_box["info"]
+            cmp_["default_sweeps"] = res_box["stats"]["sweeps"]
+            cmp_["default_aed_calls"] = res_box["stats"]["aed_calls"]
+            cmp_["default_swap_fail"] = res_box["stats"]["swap_fail"]
+            cmp_["gain"] = res_box["t"] / learned_s
+            cmp_["note"] = "same pencil, same call (JOB=S, COMPQ=COMPZ=I), PIPARMQ default parameters"
+            log("default: %.2f s (gain %.2fx)" % (res_box["t"], cmp_["gain"]))
+        else:
+            lb = time.perf_counter() - t_d0
+            cmp_["default_lower_bound_s"] = lb
+            cmp_["gain_lower_bound"] = lb / learned_s
+            cmp_["note"] = ("default-parameter run (JOB=S, COMPQ=COMPZ=I) still running when the budget "
+                            "ended: its time is at least %.1f s" % lb)
+            log(cmp_["note"])
+    em.emit()
+    sys.stdout.flush()
+    # a default run still in flight is abandoned with the process
+    os._exit(0)
 
 
 if __name__ == "__main__":

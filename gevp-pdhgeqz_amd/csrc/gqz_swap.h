@@ -1,13 +1,13 @@
 // gqz_swap.h -- register-resident swap of adjacent diagonal blocks (DTGEX2 semantics,
 // src/reorder/dtgex2.f; direct swapping method of Kagstrom and Poromaa).
 //
-// For blocks of orders N1 (upper) and N2 (lower) of the (N1+N2) x (N1+N2) corner (S, T):
-//   1. solve the generalized Sylvester equation  S11 R - L S22 = g S12,  T11 R - L T22 = g T12
-//      by Gaussian elimination with complete pivoting on its Kronecker form (DTGSY2/DGETC2),
-//   2. Za = orthogonal factor of QR([-R; g I]), Qa = orthogonal factor of QR([-L; g I]),
-//   3. S <- Qa^T S Za, T <- Qa^T T Za, with the weak (lower-left block) and strong
-//      (backward error of the reconstruction) stability tests of DTGEX2,
-//   4. re-standardise 2x2 blocks (DLAGV2).
+// For blocks of orders N1 (upper) and N2 (lower) of the (N1+N2) x (N1+N2) corner (S, T), as the
+// tree's DTGEX2 does: scale the corner by its largest entries, judge both matrices by one threshold,
+// swap two 1x1 blocks by Givens rotations, and otherwise solve the generalized Sylvester equation
+// S11 R - L S22 = g S12, T11 R - L T22 = g T12 (complete pivoting on its Kronecker form, DTGSY2 /
+// DGETC2), take the left factor from QR([-L; g I]) and the right one from RQ([g I, R]),
+// re-triangularise T by RQ or QR (whichever leaves the smaller S21), test the reconstruction and
+// standardise the new 2x2 blocks (DLAGV2).
 // Everything is templated on the block sizes and fully unrolled so that the arrays live in
 // registers (no scratch memory).
 #pragma once
@@ -182,118 +182,383 @@ __device__ __forceinline__ void r_mm(const double (&A)[M * M], bool ta, const do
         }
 }
 
-// 2x2 standardisation of the diagonal block at offset O of an M x M register pencil,
-// accumulating the rotations into Qa (left) and Za (right)
-template <int M, int O>
-__device__ __forceinline__ void r_std2(double (&S)[M * M], double (&T)[M * M], double (&Qa)[M * M],
-                                       double (&Za)[M * M]) {
-    // make T's block upper triangular first
-    double c, s, r;
-    d_lartg(T[(O + 1) + M * (O + 1)], T[(O + 1) + M * O], c, s, r);
+// Householder reflector (I - tau v v^T) of the entries x[lo..hi] of a register column, pivot at
+// index `piv` (lo or hi): v[piv] = 1, v = 0 outside [lo, hi]; beta is the new pivot entry.  tau is
+// 2 / (v^T v) of the rounded v (orthogonal to working precision whatever the rounding of v).
+template <int M>
+__device__ __forceinline__ void r_house(const double (&x)[M], int lo, int hi, int piv, double (&v)[M], double &tau,
+                                        double &beta) {
+    double ss = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) if (i >= lo && i <= hi && i != piv) ss += x[i] * x[i];
+    double alpha = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) if (i == piv) alpha = x[i];
+    if (ss == 0.0) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) v[i] = 0.0;
+        tau = 0.0; beta = alpha;
+        return;
+    }
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    const double sc = 1.0 / (alpha - beta);
+    double vv = 1.0;
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        double x, y;
-        x = S[i + M * (O + 1)]; y = S[i + M * O]; S[i + M * (O + 1)] = c * x + s * y; S[i + M * O] = c * y - s * x;
-        x = T[i + M * (O + 1)]; y = T[i + M * O]; T[i + M * (O + 1)] = c * x + s * y; T[i + M * O] = c * y - s * x;
-        x = Za[i + M * (O + 1)]; y = Za[i + M * O]; Za[i + M * (O + 1)] = c * x + s * y; Za[i + M * O] = c * y - s * x;
+        v[i] = (i == piv) ? 1.0 : ((i >= lo && i <= hi) ? x[i] * sc : 0.0);
+        if (i != piv) vv += v[i] * v[i];
     }
-    T[(O + 1) + M * O] = 0.0;
+    tau = 2.0 / vv;
+}
+
+// X (M x M) <- X (I - tau v v^T)  /  X <- (I - tau v v^T) X
+template <int M>
+__device__ __forceinline__ void r_refl_right(double (&X)[M * M], const double (&v)[M], double tau) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) t += X[i + M * j] * v[j];
+        t *= tau;
+#pragma unroll
+        for (int j = 0; j < M; ++j) X[i + M * j] -= t * v[j];
+    }
+}
+template <int M>
+__device__ __forceinline__ void r_refl_left(double (&X)[M * M], const double (&v)[M], double tau) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) t += v[i] * X[i + M * j];
+        t *= tau;
+#pragma unroll
+        for (int i = 0; i < M; ++i) X[i + M * j] -= t * v[i];
+    }
+}
+
+// Full orthogonal factor Qo (M x M) of the RQ factorisation of the K x M matrix X = [g I_K, R]
+// (DGERQ2 + DORGR2 of dtgex2.f:546-563): X Qo^T = [0 TR].  Rows are reduced bottom first.
+template <int K, int M>
+__device__ __forceinline__ void r_rq_full(double (&X)[K * M], double (&Qo)[M * M]) {
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int i = 0; i < M; ++i) Qo[i + M * j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = K - 1; r >= 0; --r) {
+        const int last = M - (K - 1 - r) - 1;       // pivot column of row r
+        double x[M], v[M], tau, beta;
+#pragma unroll
+        for (int j = 0; j < M; ++j) x[j] = X[r + K * j];
+        r_house<M>(x, 0, last, last, v, tau, beta);
+#pragma unroll
+        for (int j = 0; j < M; ++j) X[r + K * j] = (j == last) ? beta : (j < last ? 0.0 : X[r + K * j]);
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (i >= r) continue;
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) t += X[i + K * j] * v[j];
+            t *= tau;
+#pragma unroll
+            for (int j = 0; j < M; ++j) X[i + K * j] -= t * v[j];
+        }
+        r_refl_left<M>(Qo, v, tau);
+    }
+}
+
+// Re-triangularise T of a tentative swap by an RQ factorisation: T <- T Qt^T (upper triangular),
+// S <- S Qt^T, IR <- Qt IR (dtgex2.f:582-605).  Returns ||S(N2:M, 0:N2)||_F.
+template <int M, int N2>
+__device__ __forceinline__ double r_retri_rq(double (&S)[M * M], double (&T)[M * M], double (&IR)[M * M]) {
+#pragma unroll
+    for (int r = M - 1; r >= 1; --r) {
+        double x[M], v[M], tau, beta;
+#pragma unroll
+        for (int j = 0; j < M; ++j) x[j] = T[r + M * j];
+        r_house<M>(x, 0, r, r, v, tau, beta);
+        r_refl_right<M>(T, v, tau);
+#pragma unroll
+        for (int j = 0; j < M; ++j) if (j < r) T[r + M * j] = 0.0;
+        T[r + M * r] = beta;
+        r_refl_right<M>(S, v, tau);
+        r_refl_left<M>(IR, v, tau);
+    }
+    double e = 0.0;
+#pragma unroll
+    for (int j = 0; j < N2; ++j)
+#pragma unroll
+        for (int i = N2; i < M; ++i) e += S[i + M * j] * S[i + M * j];
+    return sqrt(e);
+}
+
+// ... or by a QR factorisation: T <- Qt^T T, S <- Qt^T S, LI <- LI Qt (dtgex2.f:613-636).
+template <int M, int N2>
+__device__ __forceinline__ double r_retri_qr(double (&S)[M * M], double (&T)[M * M], double (&LI)[M * M]) {
+#pragma unroll
+    for (int c = 0; c < M - 1; ++c) {
+        double x[M], v[M], tau, beta;
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = T[i + M * c];
+        r_house<M>(x, c, M - 1, c, v, tau, beta);
+        r_refl_left<M>(T, v, tau);
+#pragma unroll
+        for (int i = 0; i < M; ++i) if (i > c) T[i + M * c] = 0.0;
+        T[c + M * c] = beta;
+        r_refl_left<M>(S, v, tau);
+        r_refl_right<M>(LI, v, tau);
+    }
+    double e = 0.0;
+#pragma unroll
+    for (int j = 0; j < N2; ++j)
+#pragma unroll
+        for (int i = N2; i < M; ++i) e += S[i + M * j] * S[i + M * j];
+    return sqrt(e);
+}
+
+// DLAGV2 standardisation of the diagonal block at offset O of the M x M corner (S, T): the block
+// changes in place, its left / right 2x2 factors go to WL / WR (dtgex2.f:706-727)
+template <int M, int O>
+__device__ __forceinline__ void r_std2(double (&S)[M * M], double (&T)[M * M], double (&WL)[M * M],
+                                       double (&WR)[M * M]) {
     double Ab[4] = { S[O + M * O], S[(O + 1) + M * O], S[O + M * (O + 1)], S[(O + 1) + M * (O + 1)] };
     double Bb[4] = { T[O + M * O], 0.0, T[O + M * (O + 1)], T[(O + 1) + M * (O + 1)] };
     double QL[4], ZR[4];
     d_lagv2(Ab, Bb, QL, ZR);
-#pragma unroll
-    for (int j = 0; j < M; ++j) {   // rows O, O+1 by QL^T
-        double x = S[O + M * j], y = S[(O + 1) + M * j];
-        S[O + M * j] = QL[0] * x + QL[1] * y; S[(O + 1) + M * j] = QL[2] * x + QL[3] * y;
-        x = T[O + M * j]; y = T[(O + 1) + M * j];
-        T[O + M * j] = QL[0] * x + QL[1] * y; T[(O + 1) + M * j] = QL[2] * x + QL[3] * y;
-    }
-#pragma unroll
-    for (int i = 0; i < M; ++i) {   // columns O, O+1 by ZR; Qa by QL, Za by ZR
-        double a = S[i + M * O], b = S[i + M * (O + 1)];
-        S[i + M * O] = a * ZR[0] + b * ZR[1]; S[i + M * (O + 1)] = a * ZR[2] + b * ZR[3];
-        a = T[i + M * O]; b = T[i + M * (O + 1)];
-        T[i + M * O] = a * ZR[0] + b * ZR[1]; T[i + M * (O + 1)] = a * ZR[2] + b * ZR[3];
-        a = Qa[i + M * O]; b = Qa[i + M * (O + 1)];
-        Qa[i + M * O] = a * QL[0] + b * QL[1]; Qa[i + M * (O + 1)] = a * QL[2] + b * QL[3];
-        a = Za[i + M * O]; b = Za[i + M * (O + 1)];
-        Za[i + M * O] = a * ZR[0] + b * ZR[1]; Za[i + M * (O + 1)] = a * ZR[2] + b * ZR[3];
-    }
     S[O + M * O] = Ab[0]; S[(O + 1) + M * O] = Ab[1]; S[O + M * (O + 1)] = Ab[2]; S[(O + 1) + M * (O + 1)] = Ab[3];
     T[O + M * O] = Bb[0]; T[(O + 1) + M * O] = 0.0; T[O + M * (O + 1)] = Bb[2]; T[(O + 1) + M * (O + 1)] = Bb[3];
+    WL[O + M * O] = QL[0]; WL[(O + 1) + M * O] = QL[1]; WL[O + M * (O + 1)] = QL[2]; WL[(O + 1) + M * (O + 1)] = QL[3];
+    WR[O + M * O] = ZR[0]; WR[(O + 1) + M * O] = ZR[1]; WR[O + M * (O + 1)] = ZR[2]; WR[(O + 1) + M * (O + 1)] = ZR[3];
 }
 
-// The swap itself.  S, T: the M x M corner (in/out).  Returns 0 on success (Qa, Za set).
+// The swap itself (DTGEX2, src/reorder/dtgex2.f:221-860, the tree's modified LAPACK 3.4.2 version).
+// S, T: the M x M corner (in/out).  Returns 0 on success with S <- Qa^T S Za, T <- Qa^T T Za (the new
+// corner, exact zeros below the blocks), 1 when the swap is rejected (S, T unchanged).
+//   - the corner is scaled by its largest entries MA, MB and one THRESH = max(20 eps ||(S,T)||_F,
+//     SMLNUM) judges both matrices (:317-363);
+//   - 1x1 / 1x1: two Givens rotations, weak and strong tests, the rotations applied to the unscaled
+//     corner (:365-474);
+//   - otherwise: Sylvester equation, QR of [-L; g I] and RQ of [g I, R], tentative swap, T
+//     re-triangularised by RQ and by QR with the smaller S21 kept when it passes (:510-650), strong
+//     test (:655-686), copy back with the scales and DLAGV2 standardisation of the 2x2 blocks
+//     (:692-752).
 template <int N1, int N2>
 __device__ __forceinline__ int r_tgex2(double (&S)[(N1 + N2) * (N1 + N2)], double (&T)[(N1 + N2) * (N1 + N2)],
                                        double (&Qa)[(N1 + N2) * (N1 + N2)], double (&Za)[(N1 + N2) * (N1 + N2)]) {
     constexpr int M = N1 + N2, NN = N1 * N2, M2 = 2 * NN;
-    double S0[M * M], T0[M * M];
-    double sa = 0.0, sb = 0.0;
+    const double smlnum = GQZ_SAFMIN / GQZ_ULP, bignum = 1.0 / smlnum;
+    double ma = 0.0, mb = 0.0;
 #pragma unroll
-    for (int i = 0; i < M * M; ++i) { S0[i] = S[i]; T0[i] = T[i]; sa += S[i] * S[i]; sb += T[i] * T[i]; }
-    const double smlnum = GQZ_SAFMIN / GQZ_ULP;
-    const double tha = fmax(20.0 * GQZ_ULP * sqrt(sa), smlnum), thb = fmax(20.0 * GQZ_ULP * sqrt(sb), smlnum);
-    double K[M2 * M2], rhs[M2];
-#pragma unroll
-    for (int i = 0; i < M2 * M2; ++i) K[i] = 0.0;
-#pragma unroll
-    for (int q = 0; q < N2; ++q)
-#pragma unroll
-        for (int p = 0; p < N1; ++p) {
-            const int row = p + N1 * q;
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                K[row + M2 * (k + N1 * q)] += S0[p + M * k];
-                K[row + NN + M2 * (k + N1 * q)] += T0[p + M * k];
-            }
-#pragma unroll
-            for (int k = 0; k < N2; ++k) {
-                K[row + M2 * (NN + p + N1 * k)] -= S0[(N1 + k) + M * (N1 + q)];
-                K[row + NN + M2 * (NN + p + N1 * k)] -= T0[(N1 + k) + M * (N1 + q)];
-            }
-            rhs[row] = S0[p + M * (N1 + q)];
-            rhs[row + NN] = T0[p + M * (N1 + q)];
-        }
-    const double g = r_solve_cp<M2>(K, rhs);
-    double XR[M * N2], XL[M * N2];
-#pragma unroll
-    for (int j = 0; j < N2; ++j)
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            XR[i + M * j] = (i < N1) ? -rhs[i + N1 * j] : ((i - N1) == j ? g : 0.0);
-            XL[i + M * j] = (i < N1) ? -rhs[NN + i + N1 * j] : ((i - N1) == j ? g : 0.0);
-        }
-    r_qr_full<M, N2>(XR, Za);
-    r_qr_full<M, N2>(XL, Qa);
-    double W[M * M], S2[M * M], T2[M * M];
-    r_mm<M>(Qa, true, S0, false, W); r_mm<M>(W, false, Za, false, S2);
-    r_mm<M>(Qa, true, T0, false, W); r_mm<M>(W, false, Za, false, T2);
-    double ea = 0.0, eb = 0.0;
-#pragma unroll
-    for (int j = 0; j < N2; ++j)
-#pragma unroll
-        for (int i = N2; i < M; ++i) { ea += S2[i + M * j] * S2[i + M * j]; eb += T2[i + M * j] * T2[i + M * j]; }
-    if (!(sqrt(ea) <= tha && sqrt(eb) <= thb)) return 1;
-#pragma unroll
-    for (int j = 0; j < N2; ++j)
-#pragma unroll
-        for (int i = N2; i < M; ++i) { S2[i + M * j] = 0.0; T2[i + M * j] = 0.0; }
-    if constexpr (N2 == 2) r_std2<M, 0>(S2, T2, Qa, Za);
-    if constexpr (N1 == 2) r_std2<M, N2>(S2, T2, Qa, Za);
-    double S3[M * M], T3[M * M];
-    r_mm<M>(Qa, false, S2, false, W); r_mm<M>(W, false, Za, true, S3);
-    r_mm<M>(Qa, false, T2, false, W); r_mm<M>(W, false, Za, true, T3);
-    double da = 0.0, db = 0.0;
-#pragma unroll
-    for (int i = 0; i < M * M; ++i) { da += (S3[i] - S0[i]) * (S3[i] - S0[i]); db += (T3[i] - T0[i]) * (T3[i] - T0[i]); }
-    if (!(sqrt(da) <= tha && sqrt(db) <= thb)) return 1;
+    for (int i = 0; i < M * M; ++i) { ma = fmax(ma, fabs(S[i])); mb = fmax(mb, fabs(T[i])); }
+    ma = fmin(ma, bignum); mb = fmin(mb, bignum);
+    double Ss[M * M], Ts[M * M];      // scaled corner
+    if (ma > 0.0) ma = fmax(ma, smlnum);
+    if (mb > 0.0) mb = fmax(mb, smlnum);
+    const double rma = ma > 0.0 ? 1.0 / ma : 1.0, rmb = mb > 0.0 ? 1.0 / mb : 1.0;
+    double dn = 0.0;
 #pragma unroll
     for (int j = 0; j < M; ++j)
 #pragma unroll
-        for (int i = j + 1; i < M; ++i) T2[i + M * j] = 0.0;
+        for (int i = 0; i < M; ++i) {
+            Ss[i + M * j] = ma > 0.0 ? S[i + M * j] / ma : S[i + M * j];
+            Ts[i + M * j] = (mb > 0.0 && i <= j) ? T[i + M * j] / mb : T[i + M * j];
+            dn += Ss[i + M * j] * Ss[i + M * j] + Ts[i + M * j] * Ts[i + M * j];
+        }
+    const double thresh = fmax(20.0 * GQZ_ULP * sqrt(dn), smlnum);
+    if constexpr (M == 2) {
+        // CASE 1: Givens rotations (dtgex2.f:365-395)
+        const double f = Ss[3] * Ts[0] - Ts[3] * Ss[0];
+        const double g = Ss[3] * Ts[2] - Ts[3] * Ss[2];
+        const double sb = fabs(Ts[3]), sa = fabs(Ss[3]);
+        double cs, sn, r;
+        d_lartg(f, g, cs, sn, r);
+        const double rc = sn, rs = -cs;               // DROT(c = IR(1,1), s = IR(2,1)) on columns
 #pragma unroll
-    for (int i = 0; i < M * M; ++i) { S[i] = S2[i]; T[i] = T2[i]; }
-    return 0;
+        for (int i = 0; i < 2; ++i) {
+            double x = Ss[i], y = Ss[i + 2];
+            Ss[i] = rc * x + rs * y; Ss[i + 2] = rc * y - rs * x;
+            x = Ts[i]; y = Ts[i + 2];
+            Ts[i] = rc * x + rs * y; Ts[i + 2] = rc * y - rs * x;
+        }
+        double lc, ls;
+        if (sa >= sb) d_lartg(Ss[0], Ss[1], lc, ls, r);
+        else d_lartg(Ts[0], Ts[1], lc, ls, r);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double x = Ss[2 * j], y = Ss[1 + 2 * j];
+            Ss[2 * j] = lc * x + ls * y; Ss[1 + 2 * j] = lc * y - ls * x;
+            x = Ts[2 * j]; y = Ts[1 + 2 * j];
+            Ts[2 * j] = lc * x + ls * y; Ts[1 + 2 * j] = lc * y - ls * x;
+        }
+        // weak stability test (dtgex2.f:401-407)
+        if (!(fabs(Ss[1]) + fabs(Ts[1]) <= thresh)) return 1;
+        // strong stability test on the reconstruction LI S IR^T (dtgex2.f:409-445)
+        const double LI[4] = { lc, ls, -ls, lc }, IR[4] = { sn, -cs, cs, sn };
+        double ss = 0.0;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const double *X = t ? Ts : Ss;
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int l = 0; l < 2; ++l) a += LI[i + 2 * k] * X[k + 2 * l] * IR[j + 2 * l];
+                    const double o = t ? (mb > 0.0 ? T[i + 2 * j] / mb : T[i + 2 * j])
+                                       : (ma > 0.0 ? S[i + 2 * j] / ma : S[i + 2 * j]);
+                    ss += (o - a) * (o - a);
+                }
+        }
+        if (!(sqrt(ss) <= thresh)) return 1;
+        // the rotations applied to the unscaled corner (dtgex2.f:451-464)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            double x = S[i], y = S[i + 2];
+            S[i] = rc * x + rs * y; S[i + 2] = rc * y - rs * x;
+            x = T[i]; y = T[i + 2];
+            T[i] = rc * x + rs * y; T[i + 2] = rc * y - rs * x;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double x = S[2 * j], y = S[1 + 2 * j];
+            S[2 * j] = lc * x + ls * y; S[1 + 2 * j] = lc * y - ls * x;
+            x = T[2 * j]; y = T[1 + 2 * j];
+            T[2 * j] = lc * x + ls * y; T[1 + 2 * j] = lc * y - ls * x;
+        }
+        S[1] = 0.0; T[1] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { Qa[i] = LI[i]; Za[i] = IR[i]; }
+        (void)rma; (void)rmb;
+        return 0;
+    } else {
+        // CASE 2: generalized Sylvester equation on the scaled corner (DTGSY2: unknowns vec(R), vec(L))
+        double K[M2 * M2], rhs[M2];
+#pragma unroll
+        for (int i = 0; i < M2 * M2; ++i) K[i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < N2; ++q)
+#pragma unroll
+            for (int p = 0; p < N1; ++p) {
+                const int row = p + N1 * q;
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    K[row + M2 * (k + N1 * q)] += Ss[p + M * k];
+                    K[row + NN + M2 * (k + N1 * q)] += Ts[p + M * k];
+                }
+#pragma unroll
+                for (int k = 0; k < N2; ++k) {
+                    K[row + M2 * (NN + p + N1 * k)] -= Ss[(N1 + k) + M * (N1 + q)];
+                    K[row + NN + M2 * (NN + p + N1 * k)] -= Ts[(N1 + k) + M * (N1 + q)];
+                }
+                rhs[row] = Ss[p + M * (N1 + q)];
+                rhs[row + NN] = Ts[p + M * (N1 + q)];
+            }
+        const double g = r_solve_cp<M2>(K, rhs);
+        // LI: QR of [-L; g I_N2]; IR: RQ of [g I_N1, R] (dtgex2.f:523-563)
+        double LI[M * M], IR[M * M];
+        {
+            double XL[M * N2];
+#pragma unroll
+            for (int j = 0; j < N2; ++j)
+#pragma unroll
+                for (int i = 0; i < M; ++i) XL[i + M * j] = (i < N1) ? -rhs[NN + i + N1 * j] : ((i - N1) == j ? g : 0.0);
+            r_qr_full<M, N2>(XL, LI);
+            double XR[N1 * M];
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) XR[i + N1 * j] = (j < N1) ? (i == j ? g : 0.0) : rhs[i + N1 * (j - N1)];
+            r_rq_full<N1, M>(XR, IR);
+        }
+        // tentative swap S <- LI^T S IR^T, T likewise (dtgex2.f:567-574)
+        double W[M * M], St[M * M], Tt[M * M];
+        r_mm<M>(LI, true, Ss, false, W); r_mm<M>(W, false, IR, true, St);
+        r_mm<M>(LI, true, Ts, false, W); r_mm<M>(W, false, IR, true, Tt);
+        // both re-triangularisations of T (dtgex2.f:582-650): the QR one is kept when its S21 is
+        // not larger and passes THRESH, otherwise the RQ one when it passes, otherwise rejection
+        double S1[M * M], T1[M * M], LI1[M * M];
+#pragma unroll
+        for (int i = 0; i < M * M; ++i) { S1[i] = St[i]; T1[i] = Tt[i]; LI1[i] = LI[i]; }
+        const double bqr = r_retri_qr<M, N2>(S1, T1, LI1);
+        const double brq = r_retri_rq<M, N2>(St, Tt, IR);
+        if (bqr <= brq && bqr <= thresh) {
+#pragma unroll
+            for (int i = 0; i < M * M; ++i) { St[i] = S1[i]; Tt[i] = T1[i]; LI[i] = LI1[i]; }
+            // IR: the QR variant keeps the factor of the tentative swap; recompute it (the RQ pass
+            // above changed it in place)
+            double XR[N1 * M];
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) XR[i + N1 * j] = (j < N1) ? (i == j ? g : 0.0) : rhs[i + N1 * (j - N1)];
+            r_rq_full<N1, M>(XR, IR);
+        } else if (brq >= thresh) {
+            return 1;
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) if (i > j) Tt[i + M * j] = 0.0;
+        // strong stability test: ||(S/MA - LI S' IR, T/MB - LI T' IR)||_F <= THRESH (dtgex2.f:655-686)
+        {
+            double ss = 0.0;
+            r_mm<M>(LI, false, St, false, W);
+            double R3[M * M];
+            r_mm<M>(W, false, IR, false, R3);
+#pragma unroll
+            for (int i = 0; i < M * M; ++i) { const double o = S[i] * rma; ss += (o - R3[i]) * (o - R3[i]); }
+            r_mm<M>(LI, false, Tt, false, W);
+            r_mm<M>(W, false, IR, false, R3);
+#pragma unroll
+            for (int i = 0; i < M * M; ++i) { const double o = T[i] * rmb; ss += (o - R3[i]) * (o - R3[i]); }
+            if (!(sqrt(ss) <= thresh)) return 1;
+        }
+        // accepted: zero S21, copy back with the scales (dtgex2.f:692-701)
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const bool z = (j < N2 && i >= N2);
+                S[i + M * j] = z ? 0.0 : St[i + M * j] * (ma > 0.0 ? ma : 1.0);
+                T[i + M * j] = Tt[i + M * j] * (mb > 0.0 ? mb : 1.0);
+            }
+        // DLAGV2 on the new 2x2 blocks, the off-diagonal block and the factors (dtgex2.f:706-752)
+        double WL[M * M], WR[M * M];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) { WL[i + M * j] = (i == j) ? 1.0 : 0.0; WR[i + M * j] = WL[i + M * j]; }
+        if constexpr (N2 == 2) r_std2<M, 0>(S, T, WL, WR);
+        if constexpr (N1 == 2) r_std2<M, N2>(S, T, WL, WR);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            double(&X)[M * M] = t ? T : S;
+            double O[N2 * N1];
+#pragma unroll
+            for (int j = 0; j < N1; ++j)
+#pragma unroll
+                for (int i = 0; i < N2; ++i) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N2; ++k) a += WL[k + M * i] * X[k + M * (N2 + j)];
+                    O[i + N2 * j] = a;
+                }
+#pragma unroll
+            for (int j = 0; j < N1; ++j)
+#pragma unroll
+                for (int i = 0; i < N2; ++i) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N1; ++k) a += O[i + N2 * k] * WR[(N2 + k) + M * (N2 + j)];
+                    X[i + M * (N2 + j)] = a;
+                }
+        }
+        r_mm<M>(LI, false, WL, false, Qa);
+        r_mm<M>(IR, true, WR, false, Za);
+        return 0;
+    }
 }

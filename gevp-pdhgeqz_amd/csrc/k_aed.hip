@@ -19,6 +19,8 @@
 #include "gqz_swap.h"
 #include "gqz_misc.h"
 
+extern "C" void *gqz_stream(void);
+
 #define AED_MAXW 1024
 enum { ST_U = 0, ST_X = 1, ST_A = 2, ST_D = 3 };
 
@@ -44,7 +46,11 @@ __device__ __forceinline__ void pair_swap(const double *Hw, const double *Tw, in
 }
 
 // Block structure lives in LDS: bsz[p] = order of the block starting at row p (0 inside a 2x2
-// block), st[p] / tgt[p] = state and target row of that block.  One round:
+// block), st[p] / tgt[p] = state and target row of that block.  out[2] = 0 (done), 1 (the window is
+// not quasi-triangular: nothing was swapped) or 2 (a swap was rejected: the round's other swaps are
+// in place, the rejected pair wrote nothing, and every row down to the bottom of the unexamined
+// region counts as undeflatable, kbc = s_kb + 1, as QZEARLY goes on to the HT restore after a
+// failed DTGEXC, qzearly.f:318-326).  One round:
 //   test (lane 0)  -> pair detection (all lanes, LDS) -> swap factors (one lane per pair)
 //   -> rows of every pair -> columns of every pair -> block-structure update (one lane per pair).
 template <int MAXW>
@@ -185,7 +191,7 @@ __device__ void aed_reorder_body(double *Hw, double *Tw, double *U, double *V, i
         // ---- block-structure update (one lane per pair; pairs are disjoint) ----
         for (int q = tid; q < np; q += nt) {
             const double *rec = pscr + (size_t)PREC * q;
-            if (rec[64] != 0.0) { s_fail = 1; continue; }
+            if (rec[64] != 0.0) { s_fail = 2; continue; }
             const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
             const int t = tgt[j1 + n1];
             atomicAdd(&s_swaps, 1);
@@ -212,7 +218,7 @@ __device__ void aed_reorder_body(double *Hw, double *Tw, double *U, double *V, i
     }
     if (tid == 0) {
         int kbc = s_kb + 1;
-        if (!s_fail && kbc < s_ilst) kbc = s_ilst;
+        if (s_fail != 1 && kbc < s_ilst) kbc = s_ilst;
         out[0] = kbc; out[1] = s_swaps; out[2] = s_fail; out[3] = (s != 0.0);
     }
     __syncthreads();
@@ -403,7 +409,7 @@ extern "C" __global__ void __launch_bounds__(COOP_NT) k_aed_reorder_coop(double 
         grid.sync();
         // ---- block-structure update (every workgroup, one lane per pair) ----
         for (int q = tid; q < np; q += nt) {
-            if (pfl[q]) { s_fail = 1; continue; }
+            if (pfl[q]) { s_fail = 2; continue; }
             const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
             const int t = tgt[j1 + n1];
             atomicAdd(&s_swaps, 1);
@@ -428,7 +434,7 @@ extern "C" __global__ void __launch_bounds__(COOP_NT) k_aed_reorder_coop(double 
     }
     if (blockIdx.x == 0 && tid == 0) {
         int kbc = s_kb + 1;
-        if (!s_fail && kbc < s_ilst) kbc = s_ilst;
+        if (s_fail != 1 && kbc < s_ilst) kbc = s_ilst;
         out[0] = kbc; out[1] = s_swaps; out[2] = s_fail; out[3] = (s != 0.0);
     }
 }
@@ -617,8 +623,9 @@ extern "C" __global__ void __launch_bounds__(1024) k_aed_spike(double *Hw, doubl
 // QZ), deflation test + reordering (the train above), eigenvalues, and - when something deflated
 // or there is no spike (PDHGEQZ3.f:840) - the spike reflector with the RQ update, the HT restore
 // of the undeflatable block and the write-back of the window and of the new spike entry.
-//   out[0] = kbc, out[1] = swaps, out[2] = reorder failed, out[3] = spike present, out[4] = INFO of
-//   the window QZ (non-zero: nothing was written back).
+//   out[0] = kbc, out[1] = swaps, out[2] = reorder status (0 done, 1 window not quasi-triangular:
+//   nothing written back, 2 a swap rejected: deflations so far kept), out[3] = spike present,
+//   out[4] = INFO of the window QZ (non-zero: nothing was written back).
 // U, V (jw x jw, ld jw) receive the window's orthogonal factors for the off-window updates.
 // ---------------------------------------------------------------------------------------------
 #define AEDS_MAXW 64
@@ -660,10 +667,10 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
     }
     aed_reorder_body<AEDS_MAXW>(sH, sT, sU, sV, ld, jw, s, smlnum, pscr, res);
     aed_eigs_body(sH, sT, ld, jw, war, wai, wb, tid, nt);
-    // a rejected swap (DTGEX2 INFO = 1) leaves the window's factors unreliable: the AED is then
-    // discarded (nothing written back, no deflation)
-    const int kbc = res[2] ? jw : res[0];
-    if (!res[2] && (kbc < jw || s == 0.0)) {
+    // a window that is not quasi-triangular (res[2] == 1) is discarded (nothing written back, no
+    // deflation); after a rejected swap (res[2] == 2) the deflations found so far are kept
+    const int kbc = res[2] == 1 ? jw : res[0];
+    if (res[2] != 1 && (kbc < jw || s == 0.0)) {
         if (kbc > 0 && s != 0.0) aed_spike_body(sH, sT, sU, sV, ld, jw, s, kbc, scr, sv, sw, red);
         if (kbc > 2 && s != 0.0) wg_gghrd(sH, sT, ld, jw, 0, kbc - 1, sU, sV, ld, jw, sh);
         __syncthreads();
@@ -679,7 +686,7 @@ extern "C" __global__ void __launch_bounds__(AEDS_NT) k_aed_small(double *H, dou
         // converged blocks across the deflation boundary for good
         if (tid == 0 && s != 0.0) H[GQZ_IX(kwtop, kwtop - 1, ldh)] = kbc > 0 ? s * sU[0] : 0.0;
     }
-    if (tid == 0) { out[0] = kbc; out[1] = res[1]; out[2] = res[2]; out[3] = res[2] ? 1 : res[3]; out[4] = 0; }
+    if (tid == 0) { out[0] = kbc; out[1] = res[1]; out[2] = res[2]; out[3] = res[2] == 1 ? 1 : res[3]; out[4] = 0; }
 }
 
 void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwtop, int jw, int has_spike,
@@ -713,3 +720,34 @@ void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, dou
                           const double *sptr, double *scr, const int *out) {
     hipLaunchKernelGGL(k_aed_spike, dim3(1), dim3(1024), 0, st, Hw, Tw, U, V, jw, sptr, scr, out);
 }
+
+// ---- kernel-level entry point: one block swap (the pair_swap lane) for parity tests ---------------
+// out[0] = 1 when the swap is rejected; otherwise out[1..64] = Qa | Za | S | T (m x m each, ld m,
+// m = n1 + n2) with the new corner S = Qa^T S_in Za, T likewise.
+__global__ void k_tgex2_one(int n1, int n2, const double *S, const double *T, int ld, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double rec[PREC];
+    for (int i = 0; i < PREC; ++i) rec[i] = 0.0;
+    if (n1 == 1 && n2 == 1) pair_swap<1, 1>(S, T, ld, 0, rec);
+    else if (n1 == 1 && n2 == 2) pair_swap<1, 2>(S, T, ld, 0, rec);
+    else if (n1 == 2 && n2 == 1) pair_swap<2, 1>(S, T, ld, 0, rec);
+    else pair_swap<2, 2>(S, T, ld, 0, rec);
+    out[0] = rec[64];
+    for (int i = 0; i < 64; ++i) out[1 + i] = rec[i];
+}
+
+extern "C" int gqz_tgex2_dev(int n1, int n2, const double *S, const double *T, int ld, double *out) {
+    if (n1 < 1 || n1 > 2 || n2 < 1 || n2 > 2 || ld < n1 + n2 || !S || !T || !out) return -1;
+    try {
+        if (gqz_init_current()) return -100;
+        hipStream_t st = (hipStream_t)gqz_stream();
+        hipLaunchKernelGGL(k_tgex2_one, dim3(1), dim3(64), 0, st, n1, n2, S, T, ld, out);
+        GQZ_HIP(hipGetLastError());
+        GQZ_HIP(hipStreamSynchronize(st));
+        return 0;
+    } catch (const std::exception &e) {
+        gqz_set_error(e.what());
+        return -100;
+    }
+}
+

@@ -91,6 +91,7 @@ struct Lib {
     bool mlp_on = false;
     std::vector<gqz_trace_rec> trace;
     int trace_cap = 0;
+    double progress_s = 0.0;         // > 0: top-level progress lines on stderr at this interval
     std::string err;
     std::mutex mu;
 } L;
@@ -136,6 +137,7 @@ struct Run {
     // deflation history for the parameter policy (top level)
     std::vector<double> hist_nd, hist_frac;
     int last_ns = 0, last_nw = 0, last_nibble = 0, last_swept = 0, last_ndinf = 0;
+    double t_start = 0.0, t_progress = 0.0;
 };
 
 // ---- per-class kernel timing: HIP events on the library stream around each launch --------------
@@ -329,7 +331,7 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         R.st.swaps += L.h_out[17];
         R.st.swap_fail += L.h_out[18] ? 1 : 0;
         const int spike = L.h_out[19];
-        if (L.h_out[18]) {          // rejected swap: the kernel wrote nothing back; no shifts either
+        if (L.h_out[18] == 1) {     // window not quasi-triangular: nothing written back, no shifts
             L.ws.release(mk);
             ls = 0;
             if (level == 0) R.st.t_aed += now() - t0;
@@ -384,10 +386,12 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
     R.st.swaps += L.h_out[17];
     R.st.swap_fail += L.h_out[18] ? 1 : 0;
     int spike = L.h_out[19];
-    if (L.h_out[18]) {
-        // a rejected swap (DTGEX2 INFO = 1) leaves the window's factors unreliable: the AED is
+    if (L.h_out[18] == 1) {
+        // a window that is not quasi-triangular has no block structure to reorder: the AED is
         // discarded (the window copy is not written back); no deflation and no shifts, so the
-        // sweep falls back to the eigenvalues of the bottom 2x2 block
+        // sweep falls back to the eigenvalues of the bottom 2x2 block.  After a rejected swap
+        // (status 2) the window is a consistent orthogonal equivalence and the deflations found
+        // before it are kept (qzearly.f:318-326)
         L.ws.release(mk);
         ls = 0;
         R.st.aed_calls++;
@@ -686,6 +690,16 @@ int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
             sweep(R, v, ktop, kbot, sh, ls, ns_it, level);
             swept = 1;
         }
+        if (top && L.progress_s > 0.0) {
+            const double t = now();
+            if (t - R.t_progress >= L.progress_s) {
+                R.t_progress = t;
+                std::fprintf(stderr, "[gqz] n=%d iteration %ld: active block [%d, %d], %d rows left, %ld sweeps, "
+                             "%ld AED calls, %.1f s\n", R.n_top, it, ktop, kbot, kbot - ilo + 1, R.st.sweeps,
+                             R.st.aed_calls, t - R.t_start);
+                std::fflush(stderr);
+            }
+        }
         if (top) {
             R.hist_frac.push_back((double)nd / std::max(1, jw));
             R.last_ns = ns_it; R.last_nw = nw_it; R.last_nibble = nib_it; R.last_swept = swept;
@@ -924,6 +938,7 @@ int gqz_mlp_policy(const float *params_host, int enable) {
 
 int gqz_set_policy(gqz_policy_fn fn, void *user) { L.pol = fn; L.pol_user = user; return 0; }
 int gqz_trace_enable(int cap) { L.trace.clear(); L.trace_cap = std::max(0, cap); return 0; }
+int gqz_progress(double seconds) { L.progress_s = seconds > 0.0 ? seconds : 0.0; return 0; }
 int gqz_trace_get(gqz_trace_rec *out, int cap) {
     int k = std::min<int>(cap, (int)L.trace.size());
     if (out) for (int i = 0; i < k; ++i) out[i] = L.trace[i];
@@ -950,6 +965,7 @@ int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi, dou
         if (check_init()) return -100;
         double t0 = now();
         Run R;
+        R.t_start = R.t_progress = t0;
         R.prm = prm ? *prm : gqz_params{};
         std::memset(&R.st, 0, sizeof R.st);
         R.n_top = n;

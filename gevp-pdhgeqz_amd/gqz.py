@@ -44,7 +44,7 @@ POLICY_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINT
                         C.POINTER(C.c_int), C.POINTER(C.c_int))
 
 EXPORTS = ["gqz_init", "gqz_finalize", "gqz_version", "gqz_stream", "gqz_sync", "gqz_set_policy",
-           "gqz_trace_enable", "gqz_trace_get", "gqz_last_error", "gqz_hgeqz_dev", "gqz_hgeqz",
+           "gqz_trace_enable", "gqz_trace_get", "gqz_progress", "gqz_last_error", "gqz_hgeqz_dev", "gqz_hgeqz",
            "gqz_gghrd_dev", "gqz_gghrd", "gqz_geqrb_dev", "gqz_pdhgeqz_host", "gqz_pdgghrd_host",
            "gqz_compres_dev", "gqz_batched_dev", "gqz_mlp_forward_dev", "gqz_mlp_train_step_dev"]
 
@@ -80,9 +80,11 @@ def lib():
     L.gqz_set_policy.argtypes = [POLICY_FN, vp]
     L.gqz_trace_enable.argtypes = [C.c_int]
     L.gqz_trace_get.argtypes = [C.POINTER(GqzTraceRec), C.c_int]
+    L.gqz_progress.argtypes = [C.c_double]
     L.gqz_slab_update_dev.argtypes = [C.c_char, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int]
     L.gqz_chase_window_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.gqz_tgex2_dev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp]
     L.gqz_small_qz_dev.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_double]
     L.gqz_batched_dev.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.gqz_batched_tune.argtypes = [C.c_int, vp, ip, ip, dp]
@@ -194,6 +196,22 @@ def chase_window(H, T, p, w, nb, t0, steps, intro, last, U, V, sr, si, sb):
                                _p(sr), _p(si), _p(sb))
     if r != 0:
         raise RuntimeError("gqz_chase_window_dev failed (%d): %s" % (r, L.gqz_last_error().decode()))
+
+
+def tgex2(n1, n2, S, T):
+    """Kernel-level hook: one DTGEX2 swap of the (n1+n2)-order corner (S, T) (column-major device
+    matrices).  Returns (rejected, Qa, Za, S', T') as host numpy arrays."""
+    import torch
+    L = init()
+    m = n1 + n2
+    out = torch.zeros(65, dtype=torch.float64, device=S.device)
+    _sync_torch()
+    r = L.gqz_tgex2_dev(n1, n2, _p(S), _p(T), ld(S), _p(out))
+    if r != 0:
+        raise RuntimeError("gqz_tgex2_dev failed (%d): %s" % (r, L.gqz_last_error().decode()))
+    o = out.cpu().numpy()
+    blk = lambda k: o[1 + 16 * k:1 + 16 * k + m * m].reshape(m, m, order="F")
+    return bool(o[0] != 0.0), blk(0), blk(1), blk(2), blk(3)
 
 
 def small_qz(H, T, U, V, alphar, alphai, beta, ttol):

@@ -56,6 +56,10 @@ int gqz_sync(void);
 int gqz_set_policy(gqz_policy_fn fn, void *user);
 int gqz_trace_enable(int cap);            /* record up to cap iterations (0 disables)         */
 int gqz_trace_get(gqz_trace_rec *out, int cap);   /* returns the number of records          */
+/* Progress of long QZ calls (no counterpart in the reference): with seconds > 0, a QZ call writes
+ * one line to stderr (top-level iteration, active block, sweeps and AED calls so far) at most every
+ * `seconds`; 0 turns it off (the default). */
+int gqz_progress(double seconds);
 const char *gqz_last_error(void);
 
 /* ---- QZ ----------------------------------------------------------------------------------
@@ -128,6 +132,11 @@ int gqz_slab_update_dev(char side, int m, int k, double *C, int ldc, const doubl
 int gqz_chase_window_dev(double *H, double *T, int ld, int p, int w, int nb, int t0, int steps,
                          int intro, int last, double *U, double *V, const double *sr,
                          const double *si, const double *sb);
+/* One swap of adjacent diagonal blocks of orders n1, n2 in {1, 2} (DTGEX2, src/reorder/dtgex2.f:221-860)
+ * on the (n1+n2) x (n1+n2) corner (S, T) at ld (device pointers), computed by the lane that does
+ * it inside the AED reorder.  out (65 device doubles): out[0] = 1 when the swap is rejected, else 0
+ * and out[1..64] = Qa | Za | S' | T' (m x m, ld m) with S' = Qa^T S Za, T' = Qa^T T Za. */
+int gqz_tgex2_dev(int n1, int n2, const double *S, const double *T, int ld, double *out);
 /* LDS-resident base QZ of an m x m block (m <= 64) at (H, T) with U, V (m x m, ld m) and the
  * block's eigenvalues; returns the kernel's INFO. */
 int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,

@@ -573,6 +573,54 @@ static void vigilant(qzctx *c, int k0, int k1) {
     }
 }
 
+/* Introduction of a chain of nb bulges at ktop (PDHGEQZA.f:245-384): bulge k is made from shift
+ * pair (2k, 2k+1) (QZFCOL's first column, the left reflector and INVHSE's right one) and, before the
+ * next one enters, every bulge already in the chain (oldest = bottom first) moves 3 steps; a bulge
+ * that reaches kbot is pushed off.  tail[k] (bulge k's tail column relative to ktop) and done[k]
+ * are outputs. */
+static void chain_intro(qzctx *c, int ktop, int kbot, int nb, const double *sr, const double *si,
+                        const double *sb, const chase_rng *g, int *tail, int *done) {
+    double sh[6];
+    for (int k = 0; k < nb; ++k) {
+        sh[0] = sr[2 * k]; sh[1] = si[2 * k]; sh[2] = sb[2 * k];
+        sh[3] = sr[2 * k + 1]; sh[4] = si[2 * k + 1]; sh[5] = sb[2 * k + 1];
+        bulge_intro(c, ktop, g, sh);
+        tail[k] = 0;
+        done[k] = 0;
+        if (k + 1 < nb) {
+            /* move existing bulges (bottom = oldest first) 3 steps */
+            for (int b = 0; b <= k; ++b) {
+                for (int s = 0; s < 3; ++s) {
+                    int col = ktop + tail[b];           /* reflector rows col+1..col+3 */
+                    if (done[b]) break;
+                    if (col + 3 > kbot) {
+                        if (col + 2 == kbot) bulge_pushoff(c, g);
+                        done[b] = 1; break;
+                    }
+                    double s6[6] = { sr[2 * b], si[2 * b], sb[2 * b], sr[2 * b + 1], si[2 * b + 1], sb[2 * b + 1] };
+                    bulge_step(c, col + 1, g, s6);
+                    tail[b]++;
+                }
+            }
+        }
+    }
+}
+
+/* Kernel-level entry for parity tests: introduce nb bulges at the top of a wsize x wsize window
+ * (chain_intro with ktop = 0, kbot = wsize - 1), accumulating the left / right transforms in U, V. */
+void orc_intro_window(int wsize, double *Aw, int lda, double *Bw, int ldb, double *U, double *V,
+                      int ldu, int nb, const double *sr, const double *si, const double *sb) {
+    qzctx cc; memset(&cc, 0, sizeof cc);
+    cc.n = wsize; cc.H = Aw; cc.T = Bw; cc.ldh = lda; cc.ldt = ldb;
+    cc.Q = U; cc.Z = V; cc.ldq = ldu; cc.ldz = ldu;
+    cc.smlnum = SAFMIN * ((double)wsize / ULP);
+    chase_rng g = { 0, wsize, wsize - 1 };
+    int *tail = (int *)malloc(sizeof(int) * (size_t)(nb > 0 ? nb : 1));
+    int *done = (int *)calloc((size_t)(nb > 0 ? nb : 1), sizeof(int));
+    chain_intro(&cc, 0, wsize - 1, nb, sr, si, sb, &g, tail, done);
+    free(tail); free(done);
+}
+
 /* One chain of nb = nsh/2 bulges through [ktop, kbot] (PDHGEQZ5 with NUMWIN = 1): the
  * bulges are introduced at ktop (PDHGEQZA: each new bulge after the previous ones moved 3
  * steps, bottom bulge first), then chased window by window with the bottom bulge moved
@@ -586,30 +634,7 @@ static void chain_sweep(qzctx *c, int ktop, int kbot, int nsh, const double *sr,
     /* tail column of each bulge relative to ktop; -1 = not yet introduced */
     int *tail = (int *)malloc(sizeof(int) * (size_t)nb);
     int *done = (int *)calloc((size_t)nb, sizeof(int));
-    double sh[6];
-    /* introduction: bulge k gets shifts (2k, 2k+1) */
-    for (int k = 0; k < nb; ++k) {
-        sh[0] = sr[2 * k]; sh[1] = si[2 * k]; sh[2] = sb[2 * k];
-        sh[3] = sr[2 * k + 1]; sh[4] = si[2 * k + 1]; sh[5] = sb[2 * k + 1];
-        bulge_intro(c, ktop, &g, sh);
-        tail[k] = 0;
-        if (k + 1 < nb) {
-            /* move existing bulges (bottom = oldest first) 3 steps */
-            for (int b = 0; b <= k; ++b) {
-                for (int s = 0; s < 3; ++s) {
-                    int col = ktop + tail[b];           /* reflector rows col+1..col+3 */
-                    if (done[b]) break;
-                    if (col + 3 > kbot) {
-                        if (col + 2 == kbot) bulge_pushoff(c, &g);
-                        done[b] = 1; break;
-                    }
-                    double s6[6] = { sr[2 * b], si[2 * b], sb[2 * b], sr[2 * b + 1], si[2 * b + 1], sb[2 * b + 1] };
-                    bulge_step(c, col + 1, &g, s6);
-                    tail[b]++;
-                }
-            }
-        }
-    }
+    chain_intro(c, ktop, kbot, nb, sr, si, sb, &g, tail, done);
     /* chase: bottom bulge (b = 0, oldest) moved movedist steps first, then the next */
     for (;;) {
         int any = 0;
@@ -820,121 +845,265 @@ static void mm(int m, const double *A, int ta, const double *B, int tb, double *
     }
 }
 
+/* RQ factorisation of a k x m matrix X (k <= m, ld ldx) by Householder reflectors from the right,
+ * bottom row first (DGERQ2 role): X Qo^T = [0 R] with R k x k upper triangular; Qo (m x m, ld m) is
+ * the full orthogonal factor (DORGR2 role with M = N = m, K = k). */
+static void rq_full(int k, int m, double *X, int ldx, double *Qo) {
+    for (int j = 0; j < m; ++j) for (int i = 0; i < m; ++i) Qo[i + m * j] = (i == j) ? 1.0 : 0.0;
+    for (int r = k - 1; r >= 0; --r) {
+        const int len = m - (k - 1 - r);      /* columns 0..len-1, pivot at column len-1 */
+        double al = X[r + ldx * (len - 1)], u[4], tau;
+        for (int j = 0; j < len - 1; ++j) u[j] = X[r + ldx * j];
+        orc_larfg(len, &al, u, 1, &tau);
+        double v[4];
+        for (int j = 0; j < len - 1; ++j) v[j] = u[j];
+        v[len - 1] = 1.0;
+        X[r + ldx * (len - 1)] = al;
+        for (int j = 0; j < len - 1; ++j) X[r + ldx * j] = 0.0;
+        /* rows above: X(0:r-1, 0:len-1) <- X H */
+        for (int i = 0; i < r; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < len; ++j) t += X[i + ldx * j] * v[j];
+            t *= tau;
+            for (int j = 0; j < len; ++j) X[i + ldx * j] -= t * v[j];
+        }
+        /* Qo <- H Qo (X = R H_k ... H_1 style accumulation: Qo = H_r ... applied last-first) */
+        for (int jj = 0; jj < m; ++jj) {
+            double t = 0.0;
+            for (int j = 0; j < len; ++j) t += v[j] * Qo[j + m * jj];
+            t *= tau;
+            for (int j = 0; j < len; ++j) Qo[j + m * jj] -= t * v[j];
+        }
+    }
+}
+
+/* Frobenius norm of the lower-left (m - n2) x n2 block of an m x m matrix */
+static double ll_norm(int m, int n2, const double *S) {
+    double s = 0.0;
+    for (int j = 0; j < n2; ++j) for (int i = n2; i < m; ++i) s += S[i + m * j] * S[i + m * j];
+    return sqrt(s);
+}
+
+/* DTGEX2 (src/reorder/dtgex2.f:221-860, the tree's modified LAPACK 3.4.2 version): swap the adjacent
+ * diagonal blocks of orders n1 (upper) and n2 (lower) at j1 of the pencil (A, B), updating Q and Z.
+ *   - the m x m corner (m = n1 + n2) is scaled by its largest entries MA, MB (dtgex2.f:317-345) and one
+ *     threshold THRESH = max(20 eps ||(S, T)||_F, SMLNUM) judges both matrices (:347-363);
+ *   - 1x1 / 1x1: two Givens rotations (:365-395), weak test |S21| + |T21| <= THRESH (:401-407),
+ *     strong test on the reconstruction of both blocks (:409-445), then the rotations applied to the
+ *     unscaled pencil (:451-474);
+ *   - otherwise: generalized Sylvester equation (DTGSY2), QR of [-L; scale I] (:523-538) and RQ of
+ *     [scale I, R] (:546-563), the tentative swap (:567-574), T re-triangularised both by RQ and by QR
+ *     and the variant with the smaller S21 kept when it passes THRESH (:582-647), the strong test on
+ *     the reconstruction (:655-686), the copy back with the scale factors (:697-701) and DLAGV2
+ *     standardisation of the new 2x2 blocks with the off-diagonal block, Q and Z updated (:706-800).
+ * Returns 0 when the swap was done, 1 when it was rejected (the pencil is then unchanged). */
 int orc_tgex2(int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
               double *Z, int ldz, int j1, int n1, int n2) {
-    int m = n1 + n2;
-    if (m < 2 || j1 + m > n) return 0;
-    double S[16], T[16];
+    const int m = n1 + n2;
+    if (n <= 1 || n1 <= 0 || n2 <= 0) return 0;
+    if (n1 > n || j1 + n1 >= n) return 0;
+    const double eps = ULP, smlnum = SAFMIN / eps, bignum = 1.0 / smlnum;
+    double S[16], T[16], LI[16], IR[16];
+    memset(LI, 0, sizeof LI); memset(IR, 0, sizeof IR);
     for (int j = 0; j < m; ++j) for (int i = 0; i < m; ++i) {
         S[i + m * j] = A[IX(j1 + i, j1 + j, lda)];
         T[i + m * j] = B[IX(j1 + i, j1 + j, ldb)];
     }
-    double sa = 0.0, sb = 0.0;
-    for (int i = 0; i < m * m; ++i) { sa += S[i] * S[i]; sb += T[i] * T[i]; }
-    double smlnum = SAFMIN / ULP;
-    double thresha = fmax(20.0 * ULP * sqrt(sa), smlnum);
-    double threshb = fmax(20.0 * ULP * sqrt(sb), smlnum);
-    /* generalized Sylvester: S11 R - L S22 = g S12, T11 R - L T22 = g T12 */
-    int nn = n1 * n2, mm2 = 2 * nn;
+    /* scale the local copy (dtgex2.f:317-345) */
+    double ma = 0.0, mb = 0.0;
+    for (int i = 0; i < m * m; ++i) { ma = fmax(ma, fabs(S[i])); mb = fmax(mb, fabs(T[i])); }
+    ma = fmin(ma, bignum); mb = fmin(mb, bignum);
+    if (ma > 0.0) { ma = fmax(ma, smlnum); for (int i = 0; i < m * m; ++i) S[i] /= ma; }
+    if (mb > 0.0) { mb = fmax(mb, smlnum); for (int j = 0; j < m; ++j) for (int i = 0; i <= j; ++i) T[i + m * j] /= mb; }
+    double dn = 0.0;
+    for (int i = 0; i < m * m; ++i) dn += S[i] * S[i] + T[i] * T[i];
+    const double thresh = fmax(20.0 * eps * sqrt(dn), smlnum);
+    /* the scaled copies of the original corner for the strong tests */
+    double S0[16], T0[16];
+    for (int j = 0; j < m; ++j) for (int i = 0; i < m; ++i) {
+        S0[i + m * j] = A[IX(j1 + i, j1 + j, lda)] * (ma > 0.0 ? 1.0 / ma : 1.0);
+        T0[i + m * j] = B[IX(j1 + i, j1 + j, ldb)] * (mb > 0.0 ? 1.0 / mb : 1.0);
+    }
+    double W[16], R3[16];
+    if (m == 2) {
+        /* CASE 1: 1x1 and 1x1 by Givens rotations (dtgex2.f:365-395) */
+        const double f = S[3] * T[0] - T[3] * S[0];
+        const double g = S[3] * T[2] - T[3] * S[2];
+        const double sb = fabs(T[3]), sa = fabs(S[3]);
+        double cs, sn, r;
+        orc_lartg(f, g, &cs, &sn, &r);
+        IR[0 + 2 * 1] = cs; IR[0] = sn; IR[1] = -cs; IR[1 + 2 * 1] = sn;
+        const double rc = IR[0], rs = IR[1];            /* DROT(c = IR(1,1), s = IR(2,1)) */
+        rot_cols(S, 2, 0, 1, 0, 2, rc, rs);
+        rot_cols(T, 2, 0, 1, 0, 2, rc, rs);
+        double lc, ls;
+        if (sa >= sb) orc_lartg(S[0], S[1], &lc, &ls, &r);
+        else orc_lartg(T[0], T[1], &lc, &ls, &r);
+        rot_rows(S, 2, 0, 1, 0, 2, lc, ls);
+        rot_rows(T, 2, 0, 1, 0, 2, lc, ls);
+        LI[0] = lc; LI[1] = ls; LI[3] = lc; LI[2] = -ls;
+        /* weak stability test (dtgex2.f:401-407) */
+        if (!(fabs(S[1]) + fabs(T[1]) <= thresh)) return 1;
+        /* strong stability test: ||(A - LI S IR^T, B - LI T IR^T)||_F / scales (dtgex2.f:409-445) */
+        double ss = 0.0;
+        mm(2, LI, 0, S, 0, W); mm(2, W, 0, IR, 1, R3);
+        for (int i = 0; i < 4; ++i) ss += (S0[i] - R3[i]) * (S0[i] - R3[i]);
+        mm(2, LI, 0, T, 0, W); mm(2, W, 0, IR, 1, R3);
+        for (int i = 0; i < 4; ++i) ss += (T0[i] - R3[i]) * (T0[i] - R3[i]);
+        if (!(sqrt(ss) <= thresh)) return 1;
+        /* apply to the unscaled pencil (dtgex2.f:451-474) */
+        rot_cols(A, lda, j1, j1 + 1, 0, j1 + 2, rc, rs);
+        rot_cols(B, ldb, j1, j1 + 1, 0, j1 + 2, rc, rs);
+        rot_rows(A, lda, j1, j1 + 1, j1, n, lc, ls);
+        rot_rows(B, ldb, j1, j1 + 1, j1, n, lc, ls);
+        A[IX(j1 + 1, j1, lda)] = 0.0;
+        B[IX(j1 + 1, j1, ldb)] = 0.0;
+        if (Z) rot_cols(Z, ldz, j1, j1 + 1, 0, n, rc, rs);
+        if (Q) rot_cols(Q, ldq, j1, j1 + 1, 0, n, lc, ls);
+        return 0;
+    }
+    /* CASE 2: generalized Sylvester equation S11 R - L S22 = scale S12, T11 R - L T22 = scale T12
+     * (DTGSY2 with one block on each side: unknowns vec(R), vec(L); DGETC2 + DGESC2) */
+    const int nn = n1 * n2, mm2 = 2 * nn;
     double K[64], rhs[8];
     memset(K, 0, sizeof K);
-    /* unknown order: vec(R) (n1 x n2, column-major) then vec(L) */
     for (int q = 0; q < n2; ++q) for (int p = 0; p < n1; ++p) {
-        int row = p + n1 * q;
-        for (int k = 0; k < n1; ++k) {            /* (S11 R)(p,q) = sum_k S11(p,k) R(k,q) */
+        const int row = p + n1 * q;
+        for (int k = 0; k < n1; ++k) {
             K[row + mm2 * (k + n1 * q)] += S[p + m * k];
             K[row + nn + mm2 * (k + n1 * q)] += T[p + m * k];
         }
-        for (int k = 0; k < n2; ++k) {            /* (L S22)(p,q) = sum_k L(p,k) S22(k,q) */
+        for (int k = 0; k < n2; ++k) {
             K[row + mm2 * (nn + p + n1 * k)] -= S[(n1 + k) + m * (n1 + q)];
             K[row + nn + mm2 * (nn + p + n1 * k)] -= T[(n1 + k) + m * (n1 + q)];
         }
         rhs[row] = S[p + m * (n1 + q)];
         rhs[row + nn] = T[p + m * (n1 + q)];
     }
-    double g = solve_cp(mm2, K, rhs);
-    double R[4], L[4];
-    for (int i = 0; i < nn; ++i) { R[i] = rhs[i]; L[i] = rhs[nn + i]; }
-    /* Z: QR of [-R; g I_n2],  Q: QR of [-L; g I_n2]  (m x n2) */
-    double XR[16], XL[16], Zr[16], Ql[16];
-    for (int j = 0; j < n2; ++j) for (int i = 0; i < m; ++i) {
-        XR[i + m * j] = (i < n1) ? -R[i + n1 * j] : ((i - n1) == j ? g : 0.0);
-        XL[i + m * j] = (i < n1) ? -L[i + n1 * j] : ((i - n1) == j ? g : 0.0);
+    const double scale = solve_cp(mm2, K, rhs);
+    /* LI: full orthogonal factor of the QR of [-L; scale I_n2] (m x n2) (dtgex2.f:523-538) */
+    double X[16];
+    for (int j = 0; j < n2; ++j) for (int i = 0; i < m; ++i)
+        X[i + m * j] = (i < n1) ? -rhs[nn + i + n1 * j] : ((i - n1) == j ? scale : 0.0);
+    qr_full(m, n2, X, m, LI);
+    /* IR: full orthogonal factor of the RQ of [scale I_n1, R] (n1 x m) (dtgex2.f:546-563) */
+    double Y[16];
+    for (int j = 0; j < m; ++j) for (int i = 0; i < n1; ++i)
+        Y[i + n1 * j] = (j < n1) ? (i == j ? scale : 0.0) : rhs[i + n1 * (j - n1)];
+    rq_full(n1, m, Y, n1, IR);
+    /* tentative swap S <- LI^T S IR^T, T likewise (dtgex2.f:567-574) */
+    mm(m, LI, 1, S, 0, W); mm(m, W, 0, IR, 1, S);
+    mm(m, LI, 1, T, 0, W); mm(m, W, 0, IR, 1, T);
+    double Sq[16], Tq[16], LIq[16], IRq[16];
+    memcpy(Sq, S, sizeof S); memcpy(Tq, T, sizeof T); memcpy(LIq, LI, sizeof LI); memcpy(IRq, IR, sizeof IR);
+    /* variant RQ: T = R Qt, S <- S Qt^T, IR <- Qt IR (dtgex2.f:582-605) */
+    for (int r = m - 1; r >= 1; --r) {
+        double al = T[r + m * r], u[4], tau, v[4];
+        for (int j = 0; j < r; ++j) u[j] = T[r + m * j];
+        orc_larfg(r + 1, &al, u, 1, &tau);
+        for (int j = 0; j < r; ++j) v[j] = u[j];
+        v[r] = 1.0;
+        T[r + m * r] = al;
+        for (int j = 0; j < r; ++j) T[r + m * j] = 0.0;
+        reflm_right(T, m, 0, r + 1, 0, r, v, tau);
+        reflm_right(S, m, 0, r + 1, 0, m, v, tau);
+        reflm_left(IR, m, 0, r + 1, 0, m, v, tau);
     }
-    qr_full(m, n2, XR, m, Zr);
-    qr_full(m, n2, XL, m, Ql);
-    double W[16], S2[16], T2[16];
-    mm(m, Ql, 1, S, 0, W); mm(m, W, 0, Zr, 0, S2);
-    mm(m, Ql, 1, T, 0, W); mm(m, W, 0, Zr, 0, T2);
-    /* weak stability test on the lower-left n1 x n2 block */
-    double ea = 0.0, eb = 0.0;
-    for (int j = 0; j < n2; ++j) for (int i = n2; i < m; ++i) {
-        ea += S2[i + m * j] * S2[i + m * j];
-        eb += T2[i + m * j] * T2[i + m * j];
+    const double brqa21 = ll_norm(m, n2, S);
+    /* variant QR: T = Qt R, S <- Qt^T S, LI <- LI Qt (dtgex2.f:613-636) */
+    for (int c = 0; c < m - 1; ++c) {
+        double al = Tq[c + m * c], u[4], tau, v[4];
+        for (int i = c + 1; i < m; ++i) u[i - c - 1] = Tq[i + m * c];
+        orc_larfg(m - c, &al, u, 1, &tau);
+        v[0] = 1.0;
+        for (int i = c + 1; i < m; ++i) v[i - c] = u[i - c - 1];
+        Tq[c + m * c] = al;
+        for (int i = c + 1; i < m; ++i) Tq[i + m * c] = 0.0;
+        reflm_left(Tq, m, c, m - c, c + 1, m, v, tau);
+        reflm_left(Sq, m, c, m - c, 0, m, v, tau);
+        reflm_right(LIq, m, c, m - c, 0, m, v, tau);
     }
-    if (sqrt(ea) > thresha || sqrt(eb) > threshb) return 1;
-    for (int j = 0; j < n2; ++j) for (int i = n2; i < m; ++i) { S2[i + m * j] = 0.0; T2[i + m * j] = 0.0; }
-    /* triangularise T in the new diagonal blocks with 2x2 standardisation */
-    double Qa[16], Za[16];
-    memcpy(Qa, Ql, sizeof(double) * m * m);
-    memcpy(Za, Zr, sizeof(double) * m * m);
-    int blk[2][2] = { { 0, n2 }, { n2, n1 } };
-    for (int b = 0; b < 2; ++b) {
-        int o = blk[b][0], sz = blk[b][1];
-        if (sz != 2) continue;
-        /* make T block upper triangular first (zero T(o+1,o)) with a right rotation */
-        double cs, sn, r;
-        orc_lartg(T2[(o + 1) + m * (o + 1)], T2[(o + 1) + m * o], &cs, &sn, &r);
-        rot_cols(S2, m, o + 1, o, 0, m, cs, sn);
-        rot_cols(T2, m, o + 1, o, 0, m, cs, sn);
-        rot_cols(Za, m, o + 1, o, 0, m, cs, sn);
-        T2[(o + 1) + m * o] = 0.0;
-        double Ab[4] = { S2[o + m * o], S2[(o + 1) + m * o], S2[o + m * (o + 1)], S2[(o + 1) + m * (o + 1)] };
-        double Bb[4] = { T2[o + m * o], 0.0, T2[o + m * (o + 1)], T2[(o + 1) + m * (o + 1)] };
-        double ar[2], ai[2], bt[2], QL[4], ZR[4];
-        orc_lagv2(Ab, 2, Bb, 2, ar, ai, bt, QL, ZR);
-        g2_rows(S2, m, o, 0, m, QL);
-        g2_rows(T2, m, o, 0, m, QL);
-        g2_cols(S2, m, o, 0, m, ZR);
-        g2_cols(T2, m, o, 0, m, ZR);
-        g2_cols(Qa, m, o, 0, m, QL);
-        g2_cols(Za, m, o, 0, m, ZR);
-        S2[o + m * o] = Ab[0]; S2[(o + 1) + m * o] = Ab[1]; S2[o + m * (o + 1)] = Ab[2]; S2[(o + 1) + m * (o + 1)] = Ab[3];
-        T2[o + m * o] = Bb[0]; T2[(o + 1) + m * o] = 0.0; T2[o + m * (o + 1)] = Bb[2]; T2[(o + 1) + m * (o + 1)] = Bb[3];
+    const double bqra21 = ll_norm(m, n2, Sq);
+    /* choice and weak stability test (dtgex2.f:643-650) */
+    if (bqra21 <= brqa21 && bqra21 <= thresh) {
+        memcpy(S, Sq, sizeof S); memcpy(T, Tq, sizeof T); memcpy(IR, IRq, sizeof IR); memcpy(LI, LIq, sizeof LI);
+    } else if (brqa21 >= thresh) {
+        return 1;
     }
-    /* strong stability test: || S - Qa S2 Za^T || */
-    double S3[16], T3[16];
-    mm(m, Qa, 0, S2, 0, W); mm(m, W, 0, Za, 1, S3);
-    mm(m, Qa, 0, T2, 0, W); mm(m, W, 0, Za, 1, T3);
-    double da = 0.0, db = 0.0;
-    for (int i = 0; i < m * m; ++i) { da += (S3[i] - S[i]) * (S3[i] - S[i]); db += (T3[i] - T[i]) * (T3[i] - T[i]); }
-    if (sqrt(da) > thresha || sqrt(db) > threshb) return 1;
-    for (int j = 1; j < m; ++j) for (int i = j + 1; i < m; ++i) T2[i + m * (j - 1)] = 0.0;
-    for (int i = 1; i < m; ++i) T2[i + m * (i - 1)] = 0.0;
-    /* apply to the full pencil */
-    double tmp[4];
-    for (int j = j1 + m; j < n; ++j) {     /* rows: Qa^T */
-        for (int i = 0; i < m; ++i) { double s = 0; for (int k = 0; k < m; ++k) s += Qa[k + m * i] * A[IX(j1 + k, j, lda)]; tmp[i] = s; }
-        for (int i = 0; i < m; ++i) A[IX(j1 + i, j, lda)] = tmp[i];
-        for (int i = 0; i < m; ++i) { double s = 0; for (int k = 0; k < m; ++k) s += Qa[k + m * i] * B[IX(j1 + k, j, ldb)]; tmp[i] = s; }
-        for (int i = 0; i < m; ++i) B[IX(j1 + i, j, ldb)] = tmp[i];
-    }
-    for (int i = 0; i < j1; ++i) {          /* columns: Za */
-        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += A[IX(i, j1 + k, lda)] * Za[k + m * j]; tmp[j] = s; }
-        for (int j = 0; j < m; ++j) A[IX(i, j1 + j, lda)] = tmp[j];
-        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += B[IX(i, j1 + k, ldb)] * Za[k + m * j]; tmp[j] = s; }
-        for (int j = 0; j < m; ++j) B[IX(i, j1 + j, ldb)] = tmp[j];
-    }
+    for (int j = 0; j < m; ++j) for (int i = j + 1; i < m; ++i) T[i + m * j] = 0.0;
+    /* strong stability test: ||(A - LI S IR, B - LI T IR)||_F / scales (dtgex2.f:655-686) */
+    double ss = 0.0;
+    mm(m, LI, 0, S, 0, W); mm(m, W, 0, IR, 0, R3);
+    for (int i = 0; i < m * m; ++i) ss += (S0[i] - R3[i]) * (S0[i] - R3[i]);
+    mm(m, LI, 0, T, 0, W); mm(m, W, 0, IR, 0, R3);
+    for (int i = 0; i < m * m; ++i) ss += (T0[i] - R3[i]) * (T0[i] - R3[i]);
+    if (!(sqrt(ss) <= thresh)) return 1;
+    /* accepted: zero S21, copy back with the scales (dtgex2.f:692-701) */
+    for (int j = 0; j < n2; ++j) for (int i = n2; i < m; ++i) S[i + m * j] = 0.0;
     for (int j = 0; j < m; ++j) for (int i = 0; i < m; ++i) {
-        A[IX(j1 + i, j1 + j, lda)] = S2[i + m * j];
-        B[IX(j1 + i, j1 + j, ldb)] = T2[i + m * j];
+        A[IX(j1 + i, j1 + j, lda)] = S[i + m * j] * ma;
+        B[IX(j1 + i, j1 + j, ldb)] = T[i + m * j] * mb;
+    }
+    /* standardise the new 2x2 blocks (dtgex2.f:706-727): WL (left) and WR (right) m x m */
+    double WL[16], WR[16];
+    for (int j = 0; j < m; ++j) for (int i = 0; i < m; ++i) WL[i + m * j] = WR[i + m * j] = (i == j) ? 1.0 : 0.0;
+    double ar2[2], ai2[2], be2[2], QL[4], ZR[4];
+    if (n2 > 1) {
+        orc_lagv2(&A[IX(j1, j1, lda)], lda, &B[IX(j1, j1, ldb)], ldb, ar2, ai2, be2, QL, ZR);
+        WL[0] = QL[0]; WL[1] = QL[1]; WL[m] = QL[2]; WL[m + 1] = QL[3];
+        WR[0] = ZR[0]; WR[1] = ZR[1]; WR[m] = ZR[2]; WR[m + 1] = ZR[3];
+    }
+    if (n1 > 1) {
+        orc_lagv2(&A[IX(j1 + n2, j1 + n2, lda)], lda, &B[IX(j1 + n2, j1 + n2, ldb)], ldb, ar2, ai2, be2, QL, ZR);
+        const int o = n2;
+        WL[o + m * o] = QL[0]; WL[(o + 1) + m * o] = QL[1]; WL[o + m * (o + 1)] = QL[2]; WL[(o + 1) + m * (o + 1)] = QL[3];
+        WR[o + m * o] = ZR[0]; WR[(o + 1) + m * o] = ZR[1]; WR[o + m * (o + 1)] = ZR[2]; WR[(o + 1) + m * (o + 1)] = ZR[3];
+    }
+    /* off-diagonal block A(j1:j1+n2-1, j1+n2:j1+m-1) <- WL11^T (.) WR22 (dtgex2.f:728-745) */
+    for (int t = 0; t < 2; ++t) {
+        double *M = t ? B : A;
+        const int ldm = t ? ldb : lda;
+        double O[4], O2[4];
+        for (int j = 0; j < n1; ++j) for (int i = 0; i < n2; ++i) O[i + n2 * j] = M[IX(j1 + i, j1 + n2 + j, ldm)];
+        for (int j = 0; j < n1; ++j) for (int i = 0; i < n2; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < n2; ++k) s += WL[k + m * i] * O[k + n2 * j];
+            O2[i + n2 * j] = s;
+        }
+        for (int j = 0; j < n1; ++j) for (int i = 0; i < n2; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < n1; ++k) s += O2[i + n2 * k] * WR[(n2 + k) + m * (n2 + j)];
+            M[IX(j1 + i, j1 + n2 + j, ldm)] = s;
+        }
+    }
+    /* LI <- LI WL, IR <- IR^T WR (dtgex2.f:740, :750-752): the final left and right factors */
+    mm(m, LI, 0, WL, 0, W); memcpy(LI, W, sizeof(double) * m * m);
+    mm(m, IR, 1, WR, 0, W); memcpy(IR, W, sizeof(double) * m * m);
+    /* Q <- Q LI, Z <- Z IR, rows right of the corner <- LI^T (.), columns above <- (.) IR
+     * (dtgex2.f:756-800) */
+    double tmp[4];
+    for (int j = j1 + m; j < n; ++j) {
+        for (int t = 0; t < 2; ++t) {
+            double *M = t ? B : A;
+            const int ldm = t ? ldb : lda;
+            for (int i = 0; i < m; ++i) { double s = 0; for (int k = 0; k < m; ++k) s += LI[k + m * i] * M[IX(j1 + k, j, ldm)]; tmp[i] = s; }
+            for (int i = 0; i < m; ++i) M[IX(j1 + i, j, ldm)] = tmp[i];
+        }
+    }
+    for (int i = 0; i < j1; ++i) {
+        for (int t = 0; t < 2; ++t) {
+            double *M = t ? B : A;
+            const int ldm = t ? ldb : lda;
+            for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += M[IX(i, j1 + k, ldm)] * IR[k + m * j]; tmp[j] = s; }
+            for (int j = 0; j < m; ++j) M[IX(i, j1 + j, ldm)] = tmp[j];
+        }
     }
     if (Q) for (int i = 0; i < n; ++i) {
-        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += Q[IX(i, j1 + k, ldq)] * Qa[k + m * j]; tmp[j] = s; }
+        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += Q[IX(i, j1 + k, ldq)] * LI[k + m * j]; tmp[j] = s; }
         for (int j = 0; j < m; ++j) Q[IX(i, j1 + j, ldq)] = tmp[j];
     }
     if (Z) for (int i = 0; i < n; ++i) {
-        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += Z[IX(i, j1 + k, ldz)] * Za[k + m * j]; tmp[j] = s; }
+        for (int j = 0; j < m; ++j) { double s = 0; for (int k = 0; k < m; ++k) s += Z[IX(i, j1 + k, ldz)] * IR[k + m * j]; tmp[j] = s; }
         for (int j = 0; j < m; ++j) Z[IX(i, j1 + j, ldz)] = tmp[j];
     }
     return 0;

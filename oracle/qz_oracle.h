@@ -121,6 +121,12 @@ void orc_chase_window(int wsize, double *Aw, int lda, double *Bw, int ldb, doubl
                       int ldu, int nb, int movedist, int last, const double *sr,
                       const double *si, const double *sb);
 
+/* Introduce nb bulges at the top of a wsize x wsize window (PDHGEQZA.f:245-384): bulge k from
+ * shift pair (2k, 2k+1); before each new bulge the ones already introduced (oldest first) move 3
+ * steps.  U, V (wsize x wsize, ld ldu) accumulate the left / right transforms. */
+void orc_intro_window(int wsize, double *Aw, int lda, double *Bw, int ldb, double *U, double *V,
+                      int ldu, int nb, const double *sr, const double *si, const double *sb);
+
 /* PCOMPRES (tools/PQZHLPS.f:19-319): resa = ||S - Q^T A Z||_F/||A||_F, resb likewise,
  * resq = ||I - Q^T Q||_F/(n eps), resz likewise.  out[0..3]. */
 void orc_compres(int n, const double *A, const double *B, const double *S, const double *T,

@@ -49,6 +49,7 @@ def lib():
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_chase_window.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, C.c_int, C.c_int,
                                        C.c_int, C.c_int, dp, dp, dp]
+        L.orc_intro_window.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp]
         L.orc_compres.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, C.c_int, dp]
         L.orc_fill_normal.argtypes = [C.c_uint64, C.c_size_t, dp]
         L.orc_fill_uniform.argtypes = [C.c_uint64, C.c_size_t, dp]
