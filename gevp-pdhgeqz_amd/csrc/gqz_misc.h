@@ -33,6 +33,11 @@ void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwt
                           double *pscr, double *scr, int *out);
 void gqz_launch_aed_eigs(hipStream_t st, const double *Hw, const double *Tw, int jw, double *war, double *wai,
                          double *wb);
+// eigenvalue reordering inside one diagonal window [p, p + w), w <= gqz_tgord_max_window() (k_aed.hip);
+// out: 6 ints (status, selection mask lo/hi, 2x2-block mask lo/hi, swaps); U, V: w x w, ld w
+int gqz_tgord_max_window();
+void gqz_launch_tgord_window(hipStream_t st, double *S, double *T, int ld, int p, int w, unsigned long long sel,
+                             double *U, double *V, int *out);
 // spike reflector + RQ update of T (scr: 4 * jw doubles); reads kbc from out[0]
 void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
                           const double *sptr, double *scr, const int *out);

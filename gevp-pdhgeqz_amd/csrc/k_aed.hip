@@ -721,6 +721,174 @@ void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, dou
     hipLaunchKernelGGL(k_aed_spike, dim3(1), dim3(1024), 0, st, Hw, Tw, U, V, jw, sptr, scr, out);
 }
 
+// ---------------------------------------------------------------------------------------------
+// k_tgord_window: eigenvalue reordering inside one LDS-resident diagonal window (the window step of
+// PDTGORD, src/reorder/PDTGORD.f:600-1600, with its local DTGEXC/DTGEX2 swaps).  The window
+// [p, p + w) of (S, T), w <= 64, is loaded with U = V = I; selected blocks (bit r of `sel` for window
+// row r; a 2x2 block is selected when its first row is) move to the top of the window: every round
+// swaps all pairs (unselected block directly above a selected block) at once - their index sets are
+// disjoint, rows are updated before columns - until no such pair is left.  The window and U, V
+// (w x w, ld w) are written back; the parts of the pencil outside the window are updated by the
+// caller with U (rows right of the window, Q) and V (columns above it, Z).
+//   out[0] = 0, or 1 + the window row of the pair whose swap was rejected (the window is then
+//            partially reordered and consistent: the rejected pair wrote nothing);
+//   out[1], out[2] = selection mask after the reordering (low, high 32 bits);
+//   out[3], out[4] = mask of the rows r with S(r + 1, r) != 0 (2x2 blocks) after the reordering;
+//   out[5] = number of swaps.
+// ---------------------------------------------------------------------------------------------
+#define TGO_MAXW 64
+#define TGO_NT 256
+
+extern "C" __global__ void __launch_bounds__(TGO_NT) k_tgord_window(double *S, double *T, int ld, int p, int w,
+                                                                    unsigned long long sel, double *U, double *V,
+                                                                    int *out) {
+    constexpr int LDW = TGO_MAXW + 1;
+    __shared__ double sS[TGO_MAXW * LDW], sT[TGO_MAXW * LDW], sU[TGO_MAXW * LDW], sV[TGO_MAXW * LDW];
+    __shared__ double rec[TGO_MAXW / 2][PREC];
+    __shared__ unsigned char bsz[TGO_MAXW], sl[TGO_MAXW];
+    __shared__ short pj1[TGO_MAXW / 2];
+    __shared__ unsigned char pn1[TGO_MAXW / 2], pn2[TGO_MAXW / 2];
+    __shared__ int s_np, s_fail, s_swaps;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *gS = S + GQZ_IX(p, p, ld), *gT = T + GQZ_IX(p, p, ld);
+    for (int idx = tid; idx < w * w; idx += nt) {
+        const int i = idx % w, j = idx / w;
+        sS[i + LDW * j] = gS[GQZ_IX(i, j, ld)];
+        sT[i + LDW * j] = gT[GQZ_IX(i, j, ld)];
+        sU[i + LDW * j] = (i == j) ? 1.0 : 0.0;
+        sV[i + LDW * j] = (i == j) ? 1.0 : 0.0;
+    }
+    if (tid == 0) { s_fail = 0; s_swaps = 0; }
+    __syncthreads();
+    for (int i = tid; i < w; i += nt) {
+        const bool lower = (i > 0) && sS[i + LDW * (i - 1)] != 0.0;
+        const bool upper = (i + 1 < w) && sS[(i + 1) + LDW * i] != 0.0;
+        bsz[i] = lower ? 0 : (upper ? 2 : 1);
+        const int b0 = lower ? i - 1 : i;
+        // a 2x2 block is selected when either of its rows is (PDTGORD.f:512-527)
+        const bool s0 = (sel >> b0) & 1ull, s1 = upper || lower ? ((sel >> (b0 + 1)) & 1ull) : false;
+        sl[i] = (s0 || s1) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int round = 0; round < 2 * TGO_MAXW + 2; ++round) {
+        if (tid == 0) s_np = 0;
+        __syncthreads();
+        // pairs: a selected block whose upper neighbour block is unselected
+        for (int q = tid; q < w; q += nt) {
+            if (bsz[q] == 0 || sl[q] || q + bsz[q] >= w) continue;
+            const int r = q + bsz[q];
+            if (!sl[r]) continue;
+            const int k = atomicAdd(&s_np, 1);
+            pj1[k] = (short)q; pn1[k] = bsz[q]; pn2[k] = bsz[r];
+        }
+        __syncthreads();
+        const int np = s_np;
+        if (np == 0) break;
+        for (int q = tid; q < np; q += nt) {
+            const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
+            if (n1 == 1 && n2 == 1) pair_swap<1, 1>(sS, sT, LDW, j1, rec[q]);
+            else if (n1 == 1 && n2 == 2) pair_swap<1, 2>(sS, sT, LDW, j1, rec[q]);
+            else if (n1 == 2 && n2 == 1) pair_swap<2, 1>(sS, sT, LDW, j1, rec[q]);
+            else pair_swap<2, 2>(sS, sT, LDW, j1, rec[q]);
+        }
+        __syncthreads();
+        // rows of every pair (columns right of its corner) and U columns
+        for (int idx = tid; idx < np * 2 * w + np * w; idx += nt) {
+            const int q = idx < np * 2 * w ? idx / (2 * w) : (idx - np * 2 * w) / w;
+            if (rec[q][64] != 0.0) continue;
+            const int j1 = pj1[q], m = pn1[q] + pn2[q];
+            const double *Qa = rec[q];
+            double x[4], y[4];
+            if (idx < np * 2 * w) {
+                const int r = idx - q * 2 * w;
+                double *M = r < w ? sS : sT;
+                const int col = r < w ? r : r - w;
+                if (col < j1 + m) continue;
+                for (int i = 0; i < m; ++i) x[i] = M[(j1 + i) + LDW * col];
+                for (int i = 0; i < m; ++i) { double a = 0; for (int k = 0; k < m; ++k) a += Qa[k + m * i] * x[k]; y[i] = a; }
+                for (int i = 0; i < m; ++i) M[(j1 + i) + LDW * col] = y[i];
+            } else {
+                const int row = idx - np * 2 * w - q * w;
+                for (int j = 0; j < m; ++j) x[j] = sU[row + LDW * (j1 + j)];
+                for (int j = 0; j < m; ++j) { double a = 0; for (int k = 0; k < m; ++k) a += x[k] * Qa[k + m * j]; y[j] = a; }
+                for (int j = 0; j < m; ++j) sU[row + LDW * (j1 + j)] = y[j];
+            }
+        }
+        __syncthreads();
+        // columns of every pair (rows above its corner), V, and the new corners
+        for (int idx = tid; idx < np * (3 * w + 16); idx += nt) {
+            const int q = idx / (3 * w + 16), r = idx - q * (3 * w + 16);
+            if (rec[q][64] != 0.0) continue;
+            const int j1 = pj1[q], m = pn1[q] + pn2[q];
+            const double *Za = rec[q] + 16;
+            double x[4], y[4];
+            if (r >= 3 * w) {
+                const int e = r - 3 * w;
+                if (e < m * m) {
+                    const int i = e % m, j = e / m;
+                    sS[(j1 + i) + LDW * (j1 + j)] = rec[q][32 + e];
+                    sT[(j1 + i) + LDW * (j1 + j)] = rec[q][48 + e];
+                }
+                continue;
+            }
+            double *M; int row;
+            if (r < w) { M = sS; row = r; if (row >= j1) continue; }
+            else if (r < 2 * w) { M = sT; row = r - w; if (row >= j1) continue; }
+            else { M = sV; row = r - 2 * w; }
+            for (int j = 0; j < m; ++j) x[j] = M[row + LDW * (j1 + j)];
+            for (int j = 0; j < m; ++j) { double a = 0; for (int k = 0; k < m; ++k) a += x[k] * Za[k + m * j]; y[j] = a; }
+            for (int j = 0; j < m; ++j) M[row + LDW * (j1 + j)] = y[j];
+        }
+        __syncthreads();
+        // block structure and selection flags travel with the blocks
+        for (int q = tid; q < np; q += nt) {
+            if (rec[q][64] != 0.0) { atomicCAS(&s_fail, 0, 1 + pj1[q]); continue; }
+            atomicAdd(&s_swaps, 1);
+            const int j1 = pj1[q], n1 = pn1[q], n2 = pn2[q];
+            // the selected block now starts at j1 (order n2), the passed one at j1 + n2 (order n1); a
+            // 2x2 block whose eigenvalues turned real splits into two 1x1 blocks
+            const bool msplit = (n2 == 2) && sS[(j1 + 1) + LDW * j1] == 0.0;
+            const bool psplit = (n1 == 2) && sS[(j1 + n2 + 1) + LDW * (j1 + n2)] == 0.0;
+            if (n2 == 2 && !msplit) { bsz[j1] = 2; bsz[j1 + 1] = 0; }
+            else if (n2 == 2) { bsz[j1] = 1; bsz[j1 + 1] = 1; }
+            else bsz[j1] = 1;
+            for (int i = 0; i < n2; ++i) sl[j1 + i] = 1;
+            const int pb = j1 + n2;
+            if (n1 == 2 && !psplit) { bsz[pb] = 2; bsz[pb + 1] = 0; }
+            else if (n1 == 2) { bsz[pb] = 1; bsz[pb + 1] = 1; }
+            else bsz[pb] = 1;
+            for (int i = 0; i < n1; ++i) sl[pb + i] = 0;
+        }
+        __syncthreads();
+        if (s_fail) break;
+    }
+    for (int idx = tid; idx < w * w; idx += nt) {
+        const int i = idx % w, j = idx / w;
+        gS[GQZ_IX(i, j, ld)] = sS[i + LDW * j];
+        gT[GQZ_IX(i, j, ld)] = sT[i + LDW * j];
+        U[i + (size_t)w * j] = sU[i + LDW * j];
+        V[i + (size_t)w * j] = sV[i + LDW * j];
+    }
+    if (tid == 0) {
+        unsigned long long ms = 0ull, mb = 0ull;
+        for (int i = 0; i < w; ++i) {
+            if (sl[i]) ms |= 1ull << i;
+            if (i + 1 < w && sS[(i + 1) + LDW * i] != 0.0) mb |= 1ull << i;
+        }
+        out[0] = s_fail;
+        out[1] = (int)(unsigned)(ms & 0xffffffffull); out[2] = (int)(unsigned)(ms >> 32);
+        out[3] = (int)(unsigned)(mb & 0xffffffffull); out[4] = (int)(unsigned)(mb >> 32);
+        out[5] = s_swaps;
+    }
+}
+
+int gqz_tgord_max_window() { return TGO_MAXW; }
+
+void gqz_launch_tgord_window(hipStream_t st, double *S, double *T, int ld, int p, int w, unsigned long long sel,
+                             double *U, double *V, int *out) {
+    hipLaunchKernelGGL(k_tgord_window, dim3(1), dim3(TGO_NT), 0, st, S, T, ld, p, w, sel, U, V, out);
+}
+
 // ---- kernel-level entry point: one block swap (the pair_swap lane) for parity tests ---------------
 // out[0] = 1 when the swap is rejected; otherwise out[1..64] = Qa | Za | S | T (m x m each, ld m,
 // m = n1 + n2) with the new corner S = Qa^T S_in Za, T likewise.

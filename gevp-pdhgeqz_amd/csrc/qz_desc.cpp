@@ -20,6 +20,7 @@
 
 #include "../../include/gqz.h"
 #include "gqz_kernels.h"
+#include "gqz_misc.h"
 
 namespace {
 enum { DTYPE_ = 0, CTXT_, M_, N_, MB_, NB_, RSRC_, CSRC_, LLD_ };
@@ -90,6 +91,88 @@ int gqz_pdgghrd_host(char compq, char compz, int n, int ilo, int ihi, double *A,
     const char cq = "NVI"[icq], cz = "NVI"[icz];
     return gqz_gghrd_rowq(cq, cz, n, ilo, ihi, A, desca[LLD_], B, descb[LLD_], icq ? Q : nullptr,
                           icq ? descq[LLD_] : n, icz ? Z : nullptr, icz ? descz[LLD_] : n);
+}
+
+// PDTGORD (src/reorder/PDTGORD.f:1-4) on a 1x1 grid.  WANTQ, WANTZ, SELECT (INTEGER, in/out), PARA(6),
+// N, S, DESCS, T, DESCT, Q, DESCQ, Z, DESCZ, ALPHAR, ALPHAI, BETA, M, DWORK, LDWORK, IWORK, LIWORK.
+// SELECT is normalised for 2x2 blocks and M counted first (PDTGORD.f:504-527); the workspace sizes
+// are the routine's own formulas for a 1x1 grid (PDTGORD.f:548-556: LWMIN = 2N + 14 NB^2 + 8 N PARA(3)
+// with LOCr = LOCc = N, LIWMIN = 5 PARA(1) + PARA(2) PARA(3) - PARA(2)(PARA(2)+1)/2);
+// LDWORK = -1 or LIWORK = -1 is a query; too small gives INFO = -31 / -33 (the routine's codes,
+// PDTGORD.f:567-572).  INFO > 0: the 1-based row where a swap was rejected, the pencil partially
+// reordered (and SELECT showing the partial order).  The device workspace is internal.
+int gqz_tgord_dev(int wantq, int wantz, int *sel, const int *para, int n, double *S, int lds, double *T, int ldt,
+                  double *Q, int ldq, double *Z, int ldz, double *alphar, double *alphai, double *beta, int *m);
+
+int gqz_pdtgord_host(int wantq, int wantz, int *select, const int *para, int n, double *S, const int *descs,
+                     double *T, const int *desct, double *Q, const int *descq, double *Z, const int *descz,
+                     double *alphar, double *alphai, double *beta, int *m, double *dwork, int ldwork, int *iwork,
+                     int liwork) {
+    if (!descs || !para) return -7;
+    const int nb = descs[MB_];
+    const long wrk1 = 2L * n + 14L * nb * nb + 4L * n * para[2] + 2L * n * para[2] + 2L * n * para[2];
+    const long iwrk1 = 5L * para[0] + (long)para[1] * para[2] - (long)para[1] * (para[1] + 1) / 2;
+    const long lwmin = std::max(1L, wrk1), liwmin = std::max(1L, iwrk1);
+    if (ldwork == -1 || liwork == -1) {
+        if (dwork) dwork[0] = (double)lwmin;
+        if (iwork) iwork[0] = (int)liwmin;
+        return 0;
+    }
+    if (n < 0) return -5;
+    if (ldwork < lwmin) return -31;
+    if (liwork < liwmin) return -33;
+    if (!desc_1x1_ok(descs, n)) return -7;
+    if (!desc_1x1_ok(desct, n)) return -9;
+    if (wantq && !desc_1x1_ok(descq, n)) return -11;
+    if (wantz && !desc_1x1_ok(descz, n)) return -13;
+    if (m) *m = 0;
+    if (n == 0) return 0;
+    if (gqz_init_current()) return -100;
+    // device copies (ld = n), run, copy back
+    try {
+        const size_t nn = (size_t)n * n;
+        double *d = nullptr;
+        if (hipMalloc((void **)&d, sizeof(double) * (4 * nn + 3 * (size_t)n)) != hipSuccess) {
+            gqz_set_error("gqz_pdtgord_host: device allocation failed");
+            return -100;
+        }
+        double *dS = d, *dT = d + nn, *dQ = d + 2 * nn, *dZ = d + 3 * nn, *de = d + 4 * nn;
+        hipStream_t st = (hipStream_t)gqz_stream();
+        auto in = [&](double *dst, const double *src, int ld) {
+            return hipMemcpy2DAsync(dst, n * sizeof(double), src, ld * sizeof(double), n * sizeof(double), n,
+                                    hipMemcpyHostToDevice, st);
+        };
+        auto out = [&](double *dst, int ld, const double *src) {
+            return hipMemcpy2DAsync(dst, ld * sizeof(double), src, n * sizeof(double), n * sizeof(double), n,
+                                    hipMemcpyDeviceToHost, st);
+        };
+        hipError_t e = in(dS, S, descs[LLD_]);
+        if (e == hipSuccess) e = in(dT, T, desct[LLD_]);
+        if (e == hipSuccess && wantq) e = in(dQ, Q, descq[LLD_]);
+        if (e == hipSuccess && wantz) e = in(dZ, Z, descz[LLD_]);
+        int info = -100;
+        if (e == hipSuccess)
+            info = gqz_tgord_dev(wantq, wantz, select, para, n, dS, n, dT, n, wantq ? dQ : nullptr, n,
+                                 wantz ? dZ : nullptr, n, de, de + n, de + 2 * n, m);
+        if (info >= 0) {
+            e = out(S, descs[LLD_], dS);
+            if (e == hipSuccess) e = out(T, desct[LLD_], dT);
+            if (e == hipSuccess && wantq) e = out(Q, descq[LLD_], dQ);
+            if (e == hipSuccess && wantz) e = out(Z, descz[LLD_], dZ);
+            if (e == hipSuccess) e = hipMemcpyAsync(alphar, de, n * sizeof(double), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(alphai, de + n, n * sizeof(double), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(beta, de + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, st);
+        }
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipFree(d);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            gqz_set_error("gqz_pdtgord_host: HIP copy failed");
+            return -100;
+        }
+        return info;
+    } catch (...) {
+        return -100;
+    }
 }
 
 }  // extern "C"

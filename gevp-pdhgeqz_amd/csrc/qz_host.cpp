@@ -750,6 +750,7 @@ int decode(char c, const char *opts) {
 }  // namespace
 
 int gqz_init_current() { return check_init(); }
+
 void gqz_set_error(const char *msg) { L.err = msg; }
 
 
@@ -1241,6 +1242,121 @@ int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, i
             if (icq) HIPCHK(hipMemcpy2DAsync(Q, ldq * sizeof(double), dW, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
             if (icz) HIPCHK(hipMemcpy2DAsync(Z, ldz * sizeof(double), dZ, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost, L.st));
         }
+        HIPCHK(hipStreamSynchronize(L.st));
+        return info;
+    } catch (const std::exception &e) {
+        L.err = e.what();
+        return -100;
+    }
+}
+
+// ---- PDTGORD role: reorder a generalized Schur form so that selected eigenvalues lead ----------
+// Windowed reordering (the delay-and-accumulate scheme of src/reorder/PDTGORD.f:600-1600 on a 1x1
+// grid): each pass carries the next `carry` selected eigenvalues (whole blocks) below the sorted
+// top region up to it through a chain of diagonal windows of order <= `wmax`, each window bottom-up
+// from the carried blocks; inside a window the LDS kernel moves the selected blocks to its top with
+// DTGEX2 swaps and accumulates U, V, and the MFMA slab kernels apply U to the rows right of the
+// window (and Q), V to the columns above it (and Z).  Returns 0, or the 1-based row of the pair whose
+// swap was rejected (the pencil is then partially reordered, consistently), as PDTGORD's INFO.
+// sel (host, n) is updated to the final positions of the selected eigenvalues; *m is their number.
+int gqz_tgord_dev(int wantq, int wantz, int *sel, const int *para, int n, double *S, int lds, double *T, int ldt,
+                  double *Q, int ldq, double *Z, int ldz, double *alphar, double *alphai, double *beta, int *m) {
+    if (!sel && n > 0) return -3;
+    if (n < 0) return -5;
+    if (lds < std::max(1, n)) return -7;
+    if (ldt != lds) return -9;
+    if (wantq && (!Q || ldq < std::max(1, n))) return -11;
+    if (wantz && (!Z || ldz < std::max(1, n))) return -13;
+    if (m) *m = 0;
+    if (n == 0) return 0;
+    try {
+        if (check_init()) return -100;
+        const int wcap = gqz_tgord_max_window();
+        int wmax = para && para[2] > 0 ? std::min(para[2], wcap) : wcap;
+        wmax = std::max(4, wmax);
+        int carry = para && para[1] > 0 ? para[1] : wmax / 2;
+        carry = std::max(2, std::min(carry, wmax / 2));
+        // block structure: sub[i] = (S(i+1, i) != 0)
+        std::vector<double> sd(std::max(1, n - 1));
+        if (n > 1) {
+            HIPCHK(hipMemcpy2DAsync(sd.data(), sizeof(double), S + 1, (size_t)(lds + 1) * sizeof(double),
+                                    sizeof(double), n - 1, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipStreamSynchronize(L.st));
+        }
+        std::vector<char> sub(n, 0), sl(n, 0);
+        for (int i = 0; i + 1 < n; ++i) sub[i] = sd[i] != 0.0;
+        // a complex pair is selected when either of its rows is (PDTGORD.f:512-527)
+        for (int k = 0; k < n; ++k) sl[k] = sel[k] != 0;
+        for (int k = 0; k + 1 < n; ++k)
+            if (sub[k] && (sl[k] || sl[k + 1])) sl[k] = sl[k + 1] = 1;
+        int msel = 0;
+        for (int k = 0; k < n; ++k) msel += sl[k];
+        if (m) *m = msel;
+        size_t mk = L.ws.mark();
+        ensure_ws(arena_bytes(std::min(n, 64), nullptr));
+        mk = L.ws.mark();
+        double *U = L.ws.alloc((size_t)wcap * wcap), *V = L.ws.alloc((size_t)wcap * wcap);
+        Run R;
+        R.prm = gqz_params{};
+        std::memset(&R.st, 0, sizeof R.st);
+        int info = 0;
+        int ilo = 0;
+        while (ilo < n && sl[ilo]) ++ilo;
+        auto bsize = [&](int k) { return (k + 1 < n && sub[k]) ? 2 : 1; };
+        while (ilo < msel && info == 0) {
+            // the carried set: the first selected blocks at or below ilo, at most `carry` rows
+            int k = ilo, cnt = 0, last = -1;
+            while (k < n) {
+                const int bs = bsize(k);
+                if (sl[k]) {
+                    if (cnt + bs > carry && cnt > 0) break;
+                    cnt += bs; last = k + bs - 1;
+                    if (cnt >= carry) break;
+                }
+                k += bs;
+            }
+            if (cnt == 0) break;
+            int bottom = last + 1;
+            for (;;) {
+                int p = std::max(ilo, bottom - wmax);
+                if (p > ilo && sub[p - 1]) ++p;           // keep 2x2 blocks whole
+                const int w = bottom - p;
+                unsigned long long mask = 0ull;
+                for (int i = 0; i < w; ++i) if (sl[p + i]) mask |= 1ull << i;
+                gqz_launch_tgord_window(L.st, S, T, lds, p, w, mask, U, V, L.d_out + 24);
+                // far updates: rows of the window right of it (S, T) and Q by U; columns above it
+                // (S, T) and Z by V
+                GqzSlabJobs J;
+                J.njobs = 0;
+                gqz_slab_add(J, S + (size_t)p + (size_t)bottom * lds, lds, n - bottom, U, w, w);
+                gqz_slab_add(J, T + (size_t)p + (size_t)bottom * ldt, ldt, n - bottom, U, w, w);
+                slab_launch(R, J, true);
+                gqz_slab_add(J, S + (size_t)p * lds, lds, p, V, w, w);
+                gqz_slab_add(J, T + (size_t)p * ldt, ldt, p, V, w, w);
+                if (wantq) gqz_slab_add(J, Q + (size_t)p * ldq, ldq, n, U, w, w);
+                if (wantz) gqz_slab_add(J, Z + (size_t)p * ldz, ldz, n, V, w, w);
+                slab_launch(R, J, false);
+                sync_read(L.h_out + 24, L.d_out + 24, 6 * sizeof(int));
+                const int *o = L.h_out + 24;
+                const unsigned long long ms = (unsigned long long)(unsigned)o[1] | ((unsigned long long)(unsigned)o[2] << 32);
+                const unsigned long long mb = (unsigned long long)(unsigned)o[3] | ((unsigned long long)(unsigned)o[4] << 32);
+                for (int i = 0; i < w; ++i) {
+                    sl[p + i] = (ms >> i) & 1ull;
+                    if (i + 1 < w) sub[p + i] = (mb >> i) & 1ull;
+                }
+                if (o[0]) { info = p + o[0]; break; }
+                int top = 0;
+                while (top < w && sl[p + top]) ++top;
+                if (p == ilo) { ilo += top; while (ilo < n && sl[ilo]) ++ilo; break; }
+                int nb = std::min(top, carry);
+                if (nb < top && nb > 0 && sub[p + nb - 1]) --nb;     // do not cut a 2x2 block
+                if (nb <= 0) break;
+                bottom = p + nb;
+            }
+        }
+        L.ws.release(mk);
+        for (int k = 0; k < n; ++k) sel[k] = sl[k];
+        gqz_launch_finalize(L.st, S, T, wantz ? Z : nullptr, lds, ldz, n, 0, n - 1, alphar, alphai, beta);
         HIPCHK(hipStreamSynchronize(L.st));
         return info;
     } catch (const std::exception &e) {

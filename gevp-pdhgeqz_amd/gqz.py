@@ -87,6 +87,10 @@ def lib():
     L.gqz_tgex2_dev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp]
     L.gqz_small_qz_dev.argtypes = [C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_double]
     L.gqz_batched_dev.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.gqz_tgord_dev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp,
+                                C.c_int, vp, vp, vp, ip]
+    L.gqz_pdtgord_host.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                   ip, vp, C.c_int, vp, C.c_int]
     L.gqz_batched_tune.argtypes = [C.c_int, vp, ip, ip, dp]
     _LIB = L
     return L
@@ -221,6 +225,28 @@ def small_qz(H, T, U, V, alphar, alphai, beta, ttol):
     _sync_torch()
     return L.gqz_small_qz_dev(m, _p(H), _p(T), ld(H), _p(U), _p(V), _p(alphar), _p(alphai), _p(beta),
                               float(ttol))
+
+
+def tgord(select, S, T, Q=None, Z=None, para=None):
+    """Reorder the generalized Schur form (S, T) (device, column-major) so that the eigenvalues with
+    select[j] != 0 lead (PDTGORD, src/reorder/PDTGORD.f:1-4); Q, Z (device) are postmultiplied when
+    given.  Returns (info, m, select_out, alphar, alphai, beta) with the eigenvalues on the device."""
+    import numpy as np
+    import torch
+    L = init()
+    n = S.shape[0]
+    sel = np.ascontiguousarray(np.asarray(select, dtype=np.int32).copy())
+    pa = None if para is None else np.ascontiguousarray(para, dtype=np.int32)
+    ar = torch.empty(n, dtype=torch.float64, device=S.device); ai = torch.empty_like(ar); be = torch.empty_like(ar)
+    m = C.c_int(0)
+    _sync_torch()
+    info = L.gqz_tgord_dev(int(Q is not None), int(Z is not None), sel.ctypes.data_as(C.c_void_p),
+                           None if pa is None else pa.ctypes.data_as(C.c_void_p), n, _p(S), ld(S), _p(T), ld(T),
+                           _p(Q), ld(Q) if Q is not None else 1, _p(Z), ld(Z) if Z is not None else 1,
+                           _p(ar), _p(ai), _p(be), C.byref(m))
+    if info == -100:
+        raise RuntimeError("gqz_tgord_dev failed: " + L.gqz_last_error().decode())
+    return info, m.value, sel, ar, ai, be
 
 
 def compres(A, B, S, T, Q, Z):

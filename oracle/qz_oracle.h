@@ -81,6 +81,11 @@ void orc_lagv2(double *A, int lda, double *B, int ldb, double alphar[2], double 
 void orc_eig22(const double *A, int lda, const double *B, int ldb, double ar[2], double ai[2],
                double b[2]);
 
+/* PDTGORD (src/reorder/PDTGORD.f:1-4) restated serially: selected eigenvalues to the top by DTGEXC
+ * moves, 2x2 blocks selected when either row is; returns 0 or the 1-based row of a rejected swap. */
+int orc_tgord(int n, double *S, int lds, double *T, int ldt, double *Q, int ldq, double *Z, int ldz,
+              int *sel, int *m);
+
 /* ---- HT reduction (PDGGHRD path) ---- */
 /* QR of B (Householder), A <- Q^T A; Q <- Q*Q_B when compq != 0 (EXRAND2.f:296-329). */
 int orc_geqrf_apply(int n, double *A, int lda, double *B, int ldb, double *Q, int ldq, int compq);

@@ -50,6 +50,8 @@ def lib():
         L.orc_chase_window.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, C.c_int, C.c_int,
                                        C.c_int, C.c_int, dp, dp, dp]
         L.orc_intro_window.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp]
+        L.orc_tgord.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.c_void_p,
+                                C.POINTER(C.c_int)]
         L.orc_compres.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, C.c_int, dp]
         L.orc_fill_normal.argtypes = [C.c_uint64, C.c_size_t, dp]
         L.orc_fill_uniform.argtypes = [C.c_uint64, C.c_size_t, dp]
@@ -116,6 +118,17 @@ def ht_reduce(A, B, compq=1, compz=1):
     lib().orc_gghrd(n, 0, n - 1, ptr(A), n, ptr(B), n, ptr(Q), n, ptr(Z), n, 2 if compq else 0,
                     2 if compz else 0)
     return A, B, Q, Z
+
+
+def tgord(S, T, Q, Z, select):
+    """oracle reorder (PDTGORD semantics): returns (S, T, Q, Z, select_out, m, info)."""
+    S, T, Q, Z = (fmat(x) for x in (S, T, Q, Z))
+    n = S.shape[0]
+    sel = np.ascontiguousarray(np.asarray(select, dtype=np.int32).copy())
+    m = C.c_int(0)
+    info = lib().orc_tgord(n, ptr(S), n, ptr(T), n, ptr(Q), n, ptr(Z), n, sel.ctypes.data_as(C.c_void_p),
+                           C.byref(m))
+    return S, T, Q, Z, sel, m.value, info
 
 
 def compres(A, B, S, T, Q, Z):

@@ -158,3 +158,89 @@ def match_known(ar, ai, be, known, A, B, fudge=100.0):
     nab = np.sqrt(np.linalg.norm(A) ** 2 + np.linalg.norm(B) ** 2)
     tol = fudge * n * EPS * nab * kk[ci]
     return float((D[ri, ci] / tol).max())
+
+
+# ---- adjacent-block swap cases (DTGEX2, src/reorder/dtgex2.f) ------------------------------------
+def swap_corner_near(n1, n2, delta, coup=1.0, seed=0):
+    """(n1 + n2)-order corner with blocks whose eigenvalues are delta apart: a 2x2 block
+    [[mu, 1], [-nu^2, mu]] (T = I) has eigenvalues mu +- i nu; the upper block sits at mu = 0.3, the
+    lower one at 0.3 + delta (or, for two 2x2 blocks, with nu = delta above and nu = 1e-3 below);
+    the off-diagonal blocks of S and T are coup * N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    m = n1 + n2
+    S = np.zeros((m, m)); T = np.zeros((m, m))
+    blk = lambda mu, nu: (np.array([[mu, 1.0], [-nu * nu, mu]]), np.eye(2))
+    if n1 == 2:
+        S[:2, :2], T[:2, :2] = blk(0.3, delta)
+    else:
+        S[0, 0], T[0, 0] = 0.3, 1.0
+    if n2 == 2:
+        S[n1:, n1:], T[n1:, n1:] = blk(0.3 + (delta if n1 == 1 else 0.0), 1e-3 if n1 == 2 else delta)
+    else:
+        S[n1, n1], T[n1, n1] = 0.3 + (delta if n1 == 1 else 0.0), 1.0
+    S[:n1, n1:] = coup * rng.standard_normal((n1, n2))
+    T[:n1, n1:] = coup * rng.standard_normal((n1, n2))
+    return S, T
+
+
+def schur_blocks(S):
+    """[(start, order)] of the diagonal blocks of a quasi-triangular S."""
+    n = S.shape[0]
+    out, j = [], 0
+    while j < n:
+        if j + 1 < n and S[j + 1, j] != 0.0:
+            out.append((j, 2)); j += 2
+        else:
+            out.append((j, 1)); j += 1
+    return out
+
+
+def random_swap_corners(count, seed, n=8):
+    """count adjacent-block corners (S, T, n1, n2) cut from the real generalized Schur forms of
+    random N(0, 1) pencils (LAPACK DGGES via scipy), cycling over the four block-size pairs."""
+    rng = np.random.default_rng(seed)
+    want = [(1, 1), (1, 2), (2, 1), (2, 2)]
+    out = []
+    while len(out) < count:
+        A = rng.standard_normal((n, n)); B = rng.standard_normal((n, n))
+        S, T, _, _ = sl.qz(A, B, output="real")
+        bl = schur_blocks(S)
+        for k in range(len(bl) - 1):
+            (j1, n1), (_, n2) = bl[k], bl[k + 1]
+            if (n1, n2) == want[len(out) % 4]:
+                m = n1 + n2
+                out.append((S[j1:j1 + m, j1:j1 + m].copy(), T[j1:j1 + m, j1:j1 + m].copy(), n1, n2))
+                break
+    return out
+
+
+def match_two(ar1, ai1, be1, ar2, ai2, be2, A, B, fudge=100.0):
+    """Worst condition-weighted chordal distance between two computed eigenvalue sets of (A, B)
+    (e.g. the GPU's and the oracle's): each eigenvalue of set 1 takes the condition number of the
+    LAPACK eigenvalue it is closest to, the sets are matched by an optimal assignment, and the
+    distance is divided by fudge * n * eps * ||(A, B)||_F * kappa."""
+    n = A.shape[0]
+    a1 = np.asarray(ar1) + 1j * np.asarray(ai1); b1 = np.asarray(be1).astype(complex)
+    a2 = np.asarray(ar2) + 1j * np.asarray(ai2); b2 = np.asarray(be2).astype(complex)
+    la, lb, cond = lapack_eigs(A, B)
+    D0 = chordal(a1[:, None], b1[:, None], la[None, :], lb[None, :])
+    r0, c0 = linear_sum_assignment(D0)
+    kap = np.empty(n); kap[r0] = cond[c0]
+    D = chordal(a1[:, None], b1[:, None], a2[None, :], b2[None, :])
+    ri, ci = linear_sum_assignment(D)
+    nab = np.sqrt(np.linalg.norm(A) ** 2 + np.linalg.norm(B) ** 2)
+    ratio = D[ri, ci] / np.maximum(fudge * n * EPS * nab * kap[ri], 1e-300)
+    return float(ratio.max())
+
+
+def diag_eigs(S, T):
+    """Eigenvalues lambda_j of a real generalized Schur form in diagonal order (complex pairs from
+    their 2x2 blocks); inf where T(j, j) = 0 for a 1x1 block."""
+    n = S.shape[0]
+    out, j = [], 0
+    while j < n:
+        if j + 1 < n and S[j + 1, j] != 0.0:
+            out += list(sl.eigvals(S[j:j + 2, j:j + 2], T[j:j + 2, j:j + 2])); j += 2
+        else:
+            out.append(S[j, j] / T[j, j] if T[j, j] != 0.0 else np.inf); j += 1
+    return np.array(out, dtype=complex)

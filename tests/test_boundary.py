@@ -261,3 +261,30 @@ def test_fortran_program_without_gpu_stops():
     r = subprocess.run([_fortran_driver(), "run", "20"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
     assert _key_values(r.stdout)["pdgghrd_info"] == ["-100"]
+
+
+def test_pdtgord_workspace_query_and_info_codes(lib):
+    """PDTGORD's descriptor surface (src/reorder/PDTGORD.f:1-4): LDWORK = -1 returns the routine's
+    1x1-grid workspace sizes (PDTGORD.f:548-556) before any device work; too small workspaces give the
+    routine's INFO codes -31 / -33 (PDTGORD.f:567-572).  No GPU needed: these paths return first."""
+    import ctypes as C
+    n, nb = 200, 60
+    para = (C.c_int * 6)(1, nb // 2, nb, 0, min(nb, 32), nb // 2)
+    d = (C.c_int * 9)(1, 0, n, n, nb, nb, 0, 0, n)
+    sel = (C.c_int * n)()
+    m = C.c_int(0)
+    dw = (C.c_double * 1)(); iw = (C.c_int * 1)()
+    vp = C.c_void_p
+    f = lib.gqz_pdtgord_host
+    f.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                  C.POINTER(C.c_int), vp, C.c_int, vp, C.c_int]
+    null = vp(0)
+    args = lambda ldw, liw: (1, 1, C.cast(sel, vp), C.cast(para, vp), n, null, C.cast(d, vp), null, C.cast(d, vp),
+                             null, C.cast(d, vp), null, C.cast(d, vp), null, null, null, C.byref(m),
+                             C.cast(dw, vp), ldw, C.cast(iw, vp), liw)
+    assert f(*args(-1, 1)) == 0
+    lw = 2 * n + 14 * nb * nb + 8 * n * nb
+    liw = 5 * 1 + (nb // 2) * nb - (nb // 2) * (nb // 2 + 1) // 2
+    assert int(dw[0]) == lw and iw[0] == liw
+    assert f(*args(lw - 1, liw)) == -31
+    assert f(*args(lw, liw - 1)) == -33

@@ -449,3 +449,111 @@ def test_oracle_ilo_ihi_subblock():
     b = slice(ilo, ihi + 1)
     ratio, _, _ = Q.match_eigs(ar[b], ai[b], be[b], H[b, b], T[b, b])
     assert ratio < 1.0
+
+
+# ---- DTGEX2 (src/reorder/dtgex2.f:221-860) as the tree ships it ------------------------------------
+def _orc_swap(S, T, n1, n2, n=None, j1=0):
+    m = S.shape[0]
+    Sf, Tf, Qf, Zf = O.fmat(S), O.fmat(T), O.fmat(np.eye(m)), O.fmat(np.eye(m))
+    r = O.lib().orc_tgex2(m, O.ptr(Sf), m, O.ptr(Tf), m, O.ptr(Qf), m, O.ptr(Zf), m, j1, n1, n2)
+    return r, Sf, Tf, Qf, Zf
+
+
+def test_tgex2_agrees_with_lapack_dtgexc():
+    """Accepted swaps of blocks cut from random generalized Schur forms: the same reordered pencil as
+    LAPACK's DTGEXC up to the orthogonal freedom inside each 2x2 block (the standardisation of a
+    complex pair is unique only up to that), i.e. Q^T Q_lapack and Z^T Z_lapack block-diagonal."""
+    from scipy.linalg import lapack
+    worst = 0.0
+    for S, T, n1, n2 in Q.random_swap_corners(200, 3):
+        m = n1 + n2
+        r, Sf, Tf, Qf, Zf = _orc_swap(S, T, n1, n2)
+        Sl, Tl, Ql, Zl, _, info = lapack.dtgexc(S, T, np.eye(m), np.eye(m), n1 + 1, 1)
+        assert r == 0 and info == 0
+        assert Q.schur_blocks(Sf) == Q.schur_blocks(Sl)
+        for M in (Qf.T @ Ql, Zf.T @ Zl):
+            E = M.copy()
+            for j, k in Q.schur_blocks(Sf):
+                E[j:j + k, j:j + k] = 0.0
+            worst = max(worst, np.abs(E).max())
+        assert np.linalg.norm(Qf @ Sf @ Zf.T - S) <= 1e-14 * np.linalg.norm(S)
+        assert np.linalg.norm(Qf @ Tf @ Zf.T - T) <= 1e-14 * np.linalg.norm(T)
+    assert worst < 1e-13
+
+
+@pytest.mark.parametrize("n1,n2", [(2, 1), (1, 2)])
+def test_tgex2_accepts_numerically_coincident_eigenvalues(n1, n2):
+    """The tree's DTGEX2 does not look at DTGSY2's INFO (dtgex2.f:510-517): a swap whose Sylvester
+    system is singular to working precision is accepted when the weak and strong tests pass.  Current
+    LAPACK leaves on that INFO and rejects the same swaps, which is what pins this rule: for eigenvalues
+    1e-9 and 1e-12 apart the oracle accepts with a backward-stable result while LAPACK rejects."""
+    from scipy.linalg import lapack
+    for d in (1e-12, 1e-9):
+        S, T = Q.swap_corner_near(n1, n2, d)
+        m = n1 + n2
+        r, Sf, Tf, Qf, Zf = _orc_swap(S, T, n1, n2)
+        assert r == 0
+        assert np.linalg.norm(Qf @ Sf @ Zf.T - S) <= 20 * Q.EPS * np.linalg.norm(S)
+        assert np.linalg.norm(Qf @ Tf @ Zf.T - T) <= 20 * Q.EPS * np.linalg.norm(T)
+        assert np.abs(Sf[n2:, :n2]).max() == 0.0 and np.abs(np.tril(Tf, -1)).max() == 0.0
+        info = lapack.dtgexc(S, T, np.eye(m), np.eye(m), n1 + 1, 1)[-1]
+        assert info == 1
+
+
+def test_tgex2_equal_1x1_eigenvalues_by_givens():
+    """Two 1x1 blocks with the same eigenvalue and an off-diagonal pair (s12, t12) not proportional
+    to (a, b): the Givens route of dtgex2.f:365-395 (f = 0, so the right rotation is exact) accepts
+    and leaves the eigenvalue pair in place with an exact reconstruction."""
+    S = np.array([[0.7, 1.3], [0.0, 0.7]]); T = np.array([[2.0, -0.4], [0.0, 2.0]])
+    r, Sf, Tf, Qf, Zf = _orc_swap(S, T, 1, 1)
+    assert r == 0
+    assert Sf[1, 0] == 0.0 and Tf[1, 0] == 0.0
+    assert abs(Sf[0, 0] / Tf[0, 0] - 0.35) < 1e-15 and abs(Sf[1, 1] / Tf[1, 1] - 0.35) < 1e-15
+    assert np.linalg.norm(Qf @ Sf @ Zf.T - S) < 1e-15 and np.linalg.norm(Qf @ Tf @ Zf.T - T) < 1e-15
+
+
+def test_tgex2_power_of_two_scaling_is_exact():
+    """The corner is scaled by its largest entries before anything is decided (dtgex2.f:317-345), so
+    scaling S by 2^k and T by 2^l scales the result bit for bit and leaves Q, Z unchanged bit for bit
+    (a rule that solves the unscaled Sylvester system or uses an unscaled joint threshold does not)."""
+    for S, T, n1, n2 in Q.random_swap_corners(24, 5):
+        r0, S0, T0, Q0, Z0 = _orc_swap(S, T, n1, n2)
+        for k, l in ((20, -30), (-40, 17)):
+            r1, S1, T1, Q1, Z1 = _orc_swap(S * 2.0 ** k, T * 2.0 ** l, n1, n2)
+            assert r1 == r0
+            if r0 == 0:
+                assert np.array_equal(S1, S0 * 2.0 ** k) and np.array_equal(T1, T0 * 2.0 ** l)
+                assert np.array_equal(Q1, Q0) and np.array_equal(Z1, Z0)
+
+
+def test_tgex2_rejection_leaves_the_pencil_unchanged():
+    """Two 2x2 blocks 1e-3 apart with O(1) coupling: both re-triangularisations leave ||S21|| about
+    2.4x above THRESH = 20 eps ||(S/MA, T/MB)||_F, so the swap is rejected (INFO = 1) and nothing
+    changes (dtgex2.f:643-650, :844-848)."""
+    S, T = Q.swap_corner_near(2, 2, 1e-3)
+    S[2:, 2:] = np.array([[0.3 + 1e-3, 1.0], [-1e-6, 0.3 + 1e-3]])
+    r, Sf, Tf, Qf, Zf = _orc_swap(S, T, 2, 2)
+    assert r == 1
+    assert np.array_equal(Sf, S) and np.array_equal(Tf, T) and np.array_equal(Qf, np.eye(4))
+
+
+def test_oracle_tgord_matches_scipy_ordqz():
+    """PDTGORD semantics (src/reorder/PDTGORD.f:1-4): the eigenvalues inside the unit disk moved to the
+    top of a real generalized Schur form.  The leading m columns of Q and Z span the deflating
+    subspaces that LAPACK's DTGSEN (scipy.linalg.ordqz) returns, the reordered pencil is equivalent to
+    the input, and SELECT comes back as the leading m positions."""
+    rng = np.random.default_rng(0)
+    for n in (40, 90):
+        A = rng.standard_normal((n, n)); B = rng.standard_normal((n, n))
+        S, T, Qm, Zm = sl.qz(A, B, output="real")
+        sel = (np.abs(Q.diag_eigs(S, T)) < 1).astype(np.int32)
+        S2, T2, Q2, Z2, so, m, info = O.tgord(S, T, Qm, Zm, sel)
+        assert info == 0 and m == int(sel.sum())
+        assert np.array_equal(so, (np.arange(n) < m).astype(np.int32))
+        ev = Q.diag_eigs(S2, T2)
+        assert (np.abs(ev[:m]) < 1).all() and (np.abs(ev[m:]) >= 1).all()
+        assert np.linalg.norm(Q2 @ S2 @ Z2.T - A) < 1e-13 * np.linalg.norm(A)
+        assert np.linalg.norm(Q2 @ T2 @ Z2.T - B) < 1e-13 * np.linalg.norm(B)
+        _, _, _, _, Qs, Zs = sl.ordqz(A, B, sort="iuc", output="real")
+        for X, Y in ((Q2, Qs), (Z2, Zs)):
+            assert np.abs(X[:, :m] @ X[:, :m].T - Y[:, :m] @ Y[:, :m].T).max() < 1e-12
