@@ -44,6 +44,8 @@ void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, dou
 
 // library state shared with entry points living beside their kernels (qz_host.cpp)
 int gqz_init_current();                 // initialise on device 0 if needed and make it current; 0 = ok
+int gqz_ready_device();                 // the initialised device made current, -1 before gqz_init (no init)
+extern "C" int gqz_batched_scratch_setup(int device);   // per-device scratch address (k_batched.hip)
 void gqz_set_error(const char *msg);    // text returned by gqz_last_error
 
 #include <stdexcept>

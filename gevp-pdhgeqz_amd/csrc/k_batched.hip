@@ -146,7 +146,24 @@ __global__ void k_randn(double *x, long n, unsigned long long seed) {
     x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+// ---- scratch of the allocation-free entry point ------------------------------------------------
+// One file-scope device array per GPU holds the work counter and the per-pencil tables (processing
+// order, orders, matrix and eigenvalue offsets) of gqz_batched_noalloc_dev.  In host code the name
+// `scratch` is only a stand-in: its device address is taken once per device with
+// hipGetSymbolAddress (gqz_batched_scratch_setup, called by gqz_init) and only that address is used.
+#define GQZ_NOALLOC_MAXC 8192
+#define NOALLOC_ORDER 256
+#define NOALLOC_N (NOALLOC_ORDER + 4 * GQZ_NOALLOC_MAXC)
+#define NOALLOC_OFF (NOALLOC_N + 4 * GQZ_NOALLOC_MAXC)
+#define NOALLOC_EOFF (NOALLOC_OFF + 8 * GQZ_NOALLOC_MAXC)
+#define NOALLOC_BYTES (NOALLOC_EOFF + 8 * GQZ_NOALLOC_MAXC)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
 namespace {
+
+#define NOALLOC_MAXDEV 64
+unsigned char *g_scratch_dev[NOALLOC_MAXDEV] = {};   // device address of `scratch` on each GPU
+unsigned char *g_stage = nullptr;                      // pinned host staging of the tables
 
 struct Batch {
     int *d_meta = nullptr;        // order[count] | n[count] | counter
@@ -246,6 +263,87 @@ int gqz_batched_dev(int count, const int *n, const long *off, const long *eoff, 
         gqz_set_error(e.what());
         return -100;
     }
+}
+
+// Allocation-free variant of gqz_batched_dev: the same kernels, launched directly (no graph, whose
+// instantiation allocates), with the counter and the tables in the per-device `scratch` array and the
+// host-side tables in the pinned staging buffer that gqz_init set up.  No hipMalloc / hipFree.
+int gqz_batched_noalloc_dev(int count, const int *n, const long *off, const long *eoff, double *A, double *Bm,
+                            double *Q, double *Z, double *alphar, double *alphai, double *beta, int *info,
+                            int wg_threads) {
+    if (count < 0) return -1;
+    if (count == 0) return 0;
+    if (count > GQZ_NOALLOC_MAXC) {
+        gqz_set_error("gqz_batched_noalloc_dev: more pencils than the allocation-free limit of 8192");
+        return -1;
+    }
+    if (!n || !off || !eoff) return -2;
+    if (!A || !Bm || !alphar || !alphai || !beta) return -5;
+    if (wg_threads != 0 && wg_threads != 128 && wg_threads != 256 && wg_threads != 512 && wg_threads != 1024)
+        return -13;
+    int maxn = 0;
+    for (int i = 0; i < count; ++i) {
+        if (n[i] < 1) return -2;
+        maxn = std::max(maxn, n[i]);
+    }
+    try {
+        const int dev = gqz_ready_device();
+        if (dev < 0 || dev >= NOALLOC_MAXDEV || !g_scratch_dev[dev] || !g_stage) {
+            gqz_set_error("gqz_batched_noalloc_dev: call gqz_init on this device first (it sets up the scratch)");
+            return -100;
+        }
+        unsigned char *scr = g_scratch_dev[dev];
+        hipStream_t st = (hipStream_t)gqz_stream();
+        int *h_order = (int *)(g_stage + NOALLOC_ORDER), *h_n = (int *)(g_stage + NOALLOC_N);
+        long *h_off = (long *)(g_stage + NOALLOC_OFF), *h_eoff = (long *)(g_stage + NOALLOC_EOFF);
+        for (int i = 0; i < count; ++i) { h_order[i] = i; h_n[i] = n[i]; h_off[i] = off[i]; h_eoff[i] = eoff[i]; }
+        // largest pencils first, ties by index (std::sort works in place)
+        std::sort(h_order, h_order + count, [&](int x, int y) { return n[x] != n[y] ? n[x] > n[y] : x < y; });
+        GQZ_HIP(hipMemcpyAsync(scr + NOALLOC_ORDER, h_order, 4 * (size_t)count, hipMemcpyHostToDevice, st));
+        GQZ_HIP(hipMemcpyAsync(scr + NOALLOC_N, h_n, 4 * (size_t)count, hipMemcpyHostToDevice, st));
+        GQZ_HIP(hipMemcpyAsync(scr + NOALLOC_OFF, h_off, 8 * (size_t)count, hipMemcpyHostToDevice, st));
+        GQZ_HIP(hipMemcpyAsync(scr + NOALLOC_EOFF, h_eoff, 8 * (size_t)count, hipMemcpyHostToDevice, st));
+        int nt, nwg;
+        default_shape(maxn, count, nt, nwg);
+        if (wg_threads) nt = wg_threads;
+        else if (B.tuned_nt) { nt = B.tuned_nt; nwg = std::min(count, B.tuned_nwg); }
+        BatchArgs a;
+        a.count = count; a.order = (const int *)(scr + NOALLOC_ORDER); a.n = (const int *)(scr + NOALLOC_N);
+        a.off = (const long *)(scr + NOALLOC_OFF); a.eoff = (const long *)(scr + NOALLOC_EOFF);
+        a.A = A; a.B = Bm; a.Q = Q; a.Z = Z; a.ar = alphar; a.ai = alphai; a.be = beta; a.info = info;
+        a.counter = (int *)scr;            // cleared by the k_zero_int launch in launch()
+        launch(st, a, nt, nwg);
+        GQZ_HIP(hipGetLastError());
+        GQZ_HIP(hipStreamSynchronize(st));
+        return 0;
+    } catch (const std::exception &e) {
+        gqz_set_error(e.what());
+        return -100;
+    }
+}
+
+// Called by gqz_init with `device` current: the device address of `scratch` on this GPU, and (once per
+// process) the pinned staging of the host-side tables.  0 on success.
+int gqz_batched_scratch_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAXDEV) return -1;
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            gqz_set_error((std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e)).c_str());
+            return -1;
+        }
+        g_scratch_dev[device] = (unsigned char *)p;
+    }
+    if (!g_stage) {
+        const hipError_t e = hipHostMalloc((void **)&g_stage, NOALLOC_BYTES, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            g_stage = nullptr;
+            gqz_set_error((std::string("hipHostMalloc (staging) failed: ") + hipGetErrorString(e)).c_str());
+            return -1;
+        }
+    }
+    return 0;
 }
 
 int gqz_batched_tune(int count, const int *n, int *best_threads, int *best_wgs, double *best_ms) {

@@ -750,7 +750,10 @@ int decode(char c, const char *opts) {
 }  // namespace
 
 int gqz_init_current() { return check_init(); }
-
+int gqz_ready_device() {
+    if (L.dev < 0) return -1;
+    return hipSetDevice(L.dev) == hipSuccess ? L.dev : -1;
+}
 void gqz_set_error(const char *msg) { L.err = msg; }
 
 
@@ -795,6 +798,7 @@ int gqz_init(int device) {
         L.h_buf_n = 3 * 8192 + 64;
         HIPCHK(hipHostMalloc((void **)&L.h_buf, L.h_buf_n * sizeof(double), hipHostMallocDefault));
         L.dev = device;
+        if (gqz_batched_scratch_setup(device) != 0) return -1;
         return 0;
     } catch (const std::exception &e) {
         L.err = e.what();

@@ -91,6 +91,7 @@ def lib():
                                 C.c_int, vp, vp, vp, ip]
     L.gqz_pdtgord_host.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                    ip, vp, C.c_int, vp, C.c_int]
+    L.gqz_batched_noalloc_dev.argtypes = L.gqz_batched_dev.argtypes
     L.gqz_batched_tune.argtypes = [C.c_int, vp, ip, ip, dp]
     _LIB = L
     return L

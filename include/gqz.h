@@ -213,6 +213,18 @@ int gqz_batched_dev(int count, const int *n, const long *off, const long *eoff,
  * wg_threads: 0 = tuned shape (or the size heuristic before tuning), else 128/256/512/1024.
  * Repeated calls with the same arguments replay a captured hipGraph. */
 
+/* Allocation-free variant of gqz_batched_dev (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same kernels and results; the work counter and the per-pencil tables
+ * live in a fixed per-GPU device array of the library (about 192 KiB, held until the process ends)
+ * instead of buffers allocated per call, and the launch is direct (no graph instantiation).
+ * Limits: count <= 8192 (larger counts return -1 with a message); gqz_init must have run on the
+ * device (it sets the scratch up; otherwise -100).  Calls that overlap on one GPU share that array and
+ * race: serialise them.  Every call resets the counter it uses. */
+int gqz_batched_noalloc_dev(int count, const int *n, const long *off, const long *eoff,
+                            double *A, double *B, double *Q, double *Z,
+                            double *alphar, double *alphai, double *beta, int *info, int wg_threads);
+
 /* Autotuner for the batched launch: times every workgroup size (128, 256, 512) on a
  * synthetic sample drawn at the quantiles of the size list n (host array) and keeps the fastest
  * for later gqz_batched_dev calls with wg_threads = 0 (sample: up to 64 pencils, orders capped at
