@@ -17,6 +17,11 @@ void gqz_launch_small(hipStream_t st, const GqzSmallJob *djobs, int njobs, int m
 void gqz_launch_small1(hipStream_t st, const GqzSmallJob &job);
 void gqz_launch_gghrd_wg(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
                          double *Z, int ldq, int nq);
+// Givens HT reduction with the rotations of each column sweep applied in bulk (k_ht_blk.hip); ws holds
+// gqz_gghrd_blk_ws_bytes(n, nq) bytes (rotation list, progress counters, per-block carries)
+size_t gqz_gghrd_blk_ws_bytes(int n, int nq);
+void gqz_launch_gghrd_blk(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
+                          double *Z, int ldq, int nq, void *ws);
 void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
                          double *tau);
 void gqz_launch_sumsq(hipStream_t st, const double *A, int ld, int m, int n, int upper, double *out);

@@ -27,6 +27,8 @@
 namespace {
 
 const double kUlp = 2.220446049250313080847e-16;
+// HT reductions of order above this use the kernels that delay and bulk-apply each sweep's rotations
+const int kHtBlockedMin = 96;
 const double kSafmin = 2.2250738585072013831e-308;
 
 struct GqzError : std::runtime_error {
@@ -409,7 +411,12 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
         }
         if (kbc > 2 && spike) {
             int kg = kt_begin(GQZ_KC_HT);
-            gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+            if (kbc > kHtBlockedMin) {
+                double *rws = L.ws.alloc(gqz_gghrd_blk_ws_bytes(jw, jw) / sizeof(double) + 1);
+                gqz_launch_gghrd_blk(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw, rws);
+            } else {
+                gqz_launch_gghrd_wg(L.st, Hw, Tw, jw, jw, 0, kbc - 1, U, V, jw, jw);
+            }
             kt_end(kg, 0.0, 32.0 * jj);
         }
         gqz_launch_window_out(L.st, v.H, v.T, v.ld, kwtop, jw, Hw, Tw, U, kwtop > ktop ? (kbc > 0 ? 1 : 2) : 0);
@@ -1041,9 +1048,18 @@ int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, in
         if (check_init()) return -100;
         if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
         if (icz == 1) gqz_launch_identity(L.st, Z, ldz, n, n);
-        if (ihi - ilo >= 2)
-            gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
-                                icq ? ldq : ldz, n);
+        if (ihi - ilo >= 2) {
+            if (ihi - ilo + 1 > kHtBlockedMin) {
+                DevBuf rws;
+                rws.alloc_async(gqz_gghrd_blk_ws_bytes(n, n), L.st);
+                gqz_launch_gghrd_blk(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
+                                     icq ? ldq : ldz, n, rws.p);
+                HIPCHK(hipStreamSynchronize(L.st));
+            } else {
+                gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
+                                    icq ? ldq : ldz, n);
+            }
+        }
         HIPCHK(hipStreamSynchronize(L.st));
         return 0;
     } catch (const std::exception &e) {

@@ -246,6 +246,34 @@ def test_ht_reduction_device(n):
     assert np.allclose(np.abs(Hn), np.abs(Ho), atol=1e-9 * np.abs(Ho).max())
 
 
+@pytest.mark.parametrize("n,ilo,ihi", [(97, 1, 97), (200, 1, 200), (513, 1, 513), (300, 21, 280)])
+def test_gghrd_blocked_matches_oracle(n, ilo, ihi):
+    """The HT reduction with delayed, bulk-applied sweep rotations (orders above 96) against the
+    oracle's unblocked Givens restatement of PDGGHRD (PDGGHRD.f:365-642): the same rotations in the
+    same order, so H, T, Q and Z agree entry by entry (rounding aside), for sizes that are not
+    multiples of the 32-position blocks and for an active sub-range [ILO, IHI]."""
+    torch, gqz, O = _mods()
+    _note("gghrd blocked", n, ilo, ihi)
+    A = O.normal(500 + n, (n, n)); B = np.triu(O.normal(600 + n, (n, n)))
+    if ilo > 1 or ihi < n:
+        # block upper triangular outside the active range, as after balancing
+        A[ihi:, :ihi] = 0.0; A[ihi:, ihi:] = np.triu(A[ihi:, ihi:])
+        A[:, :ilo - 1] = np.triu(A[:, :ilo - 1]); A[ilo - 1:, :ilo - 1] = 0.0
+    Ad, Bd = gqz.to_colmajor(torch.from_numpy(A)), gqz.to_colmajor(torch.from_numpy(B))
+    Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+    gqz.pdgghrd("I", "I", Ad, Bd, Qd, Zd, ilo=ilo, ihi=ihi)
+    H, T, Qn, Zn = _host(Ad), _host(Bd), _host(Qd), _host(Zd)
+    Ho, To, Qo, Zo = O.fmat(A), O.fmat(B), O.fmat(np.eye(n)), O.fmat(np.eye(n))
+    O.lib().orc_gghrd(n, ilo - 1, ihi - 1, O.ptr(Ho), n, O.ptr(To), n, O.ptr(Qo), n, O.ptr(Zo), n, 1, 1)
+    sc = np.abs(A).max()
+    assert np.abs(H - Ho).max() < 1e-11 * sc * n ** 0.5
+    assert np.abs(T - To).max() < 1e-11 * np.abs(B).max() * n ** 0.5
+    assert np.abs(Qn - Qo).max() < 1e-11 * n ** 0.5 and np.abs(Zn - Zo).max() < 1e-11 * n ** 0.5
+    assert np.abs(np.tril(H, -2)).max() == 0.0 and np.abs(np.tril(T, -1)).max() == 0.0
+    ra, rb, rq, rz = QC.compres(A, B, H, T, Qn, Zn)
+    assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL
+
+
 def test_pdgghrd_transposed_q_convention():
     """PDGGHRD returns Q transposed relative to PDHGEQZ (PDGGHRD.f:25-27,83-91): with COMPQ='I'
     the returned Q satisfies H = Q A Z."""
