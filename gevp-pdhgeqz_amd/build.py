@@ -22,7 +22,7 @@ ARCH = os.environ.get("GQZ_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_wgqz.hip", "k_aed.hip", "k_misc.hip", "k_gemm.hip", "k_chase.hip", "k_ht.hip", "k_batched.hip",
                "k_mlp.hip", "k_ht_blk.hip"]
-CXX_SOURCES = ["qz_host.cpp", "qz_desc.cpp"]
+CXX_SOURCES = ["qz_host.cpp", "qz_desc.cpp", "mm_io.cpp"]
 
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

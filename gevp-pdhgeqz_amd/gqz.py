@@ -91,6 +91,8 @@ def lib():
                                 C.c_int, vp, vp, vp, ip]
     L.gqz_pdtgord_host.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                    ip, vp, C.c_int, vp, C.c_int]
+    L.gqz_mm_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, ip, ip, C.POINTER(C.c_long)]
+    L.gqz_mm_read_dense.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_int]
     L.gqz_batched_noalloc_dev.argtypes = L.gqz_batched_dev.argtypes
     L.gqz_batched_tune.argtypes = [C.c_int, vp, ip, ip, dp]
     _LIB = L
@@ -248,6 +250,38 @@ def tgord(select, S, T, Q=None, Z=None, para=None):
     if info == -100:
         raise RuntimeError("gqz_tgord_dev failed: " + L.gqz_last_error().decode())
     return info, m.value, sel, ar, ai, be
+
+
+def mm_read(path):
+    """Read a Matrix Market file into a dense column-major numpy array with the library's reader
+    (mminfo2 / PDRDMM roles); a .gz file is decompressed to a temporary file first.  Returns
+    (A, info_dict)."""
+    import gzip
+    import shutil
+    import tempfile
+    import numpy as np
+    L = lib()
+    tmp = None
+    try:
+        if path.endswith(".gz"):
+            fd, tmp = tempfile.mkstemp(suffix=".mtx")
+            with os.fdopen(fd, "wb") as out, gzip.open(path, "rb") as src:
+                shutil.copyfileobj(src, out)
+            path = tmp
+        rep, field, symm = C.create_string_buffer(16), C.create_string_buffer(16), C.create_string_buffer(24)
+        m, n, nnz = C.c_int(0), C.c_int(0), C.c_long(0)
+        r = L.gqz_mm_info(path.encode(), rep, field, symm, C.byref(m), C.byref(n), C.byref(nnz))
+        if r != 0:
+            raise RuntimeError("gqz_mm_info(%s) returned %d" % (path, r))
+        A = np.zeros((m.value, n.value), order="F")
+        r = L.gqz_mm_read_dense(path.encode(), m.value, n.value, A.ctypes.data_as(C.c_void_p), max(1, m.value))
+        if r != 0:
+            raise RuntimeError("gqz_mm_read_dense(%s) returned %d" % (path, r))
+        return A, {"rep": rep.value.decode(), "field": field.value.decode(), "symm": symm.value.decode(),
+                   "nnz": nnz.value}
+    finally:
+        if tmp:
+            os.unlink(tmp)
 
 
 def compres(A, B, S, T, Q, Z):

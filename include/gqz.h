@@ -131,6 +131,18 @@ int gqz_pdtgord_host(int wantq, int wantz, int *select, const int *para, int n,
                      double *alphar, double *alphai, double *beta, int *m,
                      double *dwork, int ldwork, int *iwork, int liwork);
 
+/* ---- Matrix Market input (tools/mmio.f mminfo2 / mmread2, tools/PDRDMM.f:6-90), host only ----------
+ * gqz_mm_info: header of a .mtx file (rep: "coordinate" | "array", field: "real" | "integer" |
+ * "pattern", symm: "general" | "symmetric" | "skew-symmetric"; buffers of at least 11, 8 and 20
+ * chars, may be null).  gqz_mm_read_dense: the whole matrix into a dense column-major host array
+ * (cleared first; a symmetric file sets (i, j) and (j, i), as PDRDMM does).  Returns 0, or -1 file not
+ * readable, -2 not a Matrix Market matrix, -3 unsupported type (complex, Hermitian), -4 bad size line,
+ * -5 an entry is missing or malformed, -6 an index is out of range, -7 size differs from
+ * nrows x ncols, -8 A null or lda too small.  A Fortran caller passes the file name: Fortran unit
+ * numbers do not cross the C ABI. */
+int gqz_mm_info(const char *path, char *rep, char *field, char *symm, int *nrows, int *ncols, long *nnz);
+int gqz_mm_read_dense(const char *path, int nrows, int ncols, double *A, int lda);
+
 /* PIPARMQ tuning values (src/pdhgeqz/piparmq.f:300-531) for ISPEC 50..55 and active size nh;
  * the Fortran shim's PILAENVX answers from it. */
 int gqz_piparmq(int ispec, int nh);
