@@ -402,6 +402,24 @@ def main():
         em.out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample_n, args.seed)
         log("cpu baseline:", em.out["cpu_baseline"]["sample"])
 
+    if args.compare and args.params == "learned" and elapsed() + 3 * em.out["value"] < budget:
+        # the same pencil with the best constant window of the training runs (AED window 48 at every
+        # n, other parameters PIPARMQ's): what the network adds over a static choice
+        prm_static = gqz.GqzParams()
+        prm_static.nw = 48
+        H.copy_(H0); T.copy_(T0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, _, info_s, st_s = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm_static)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        em.out["learned_vs_static"] = {"n": n, "static_nw": 48, "static_s": ts, "static_info": info_s,
+                                       "learned_s": em.out["value"], "gain": ts / em.out["value"],
+                                       "note": "same pencil and call, AED window fixed at 48 (the fastest "
+                                               "constant window of the training runs), other parameters "
+                                               "PIPARMQ's"}
+        log("static NW=48: %.2f s (learned gain %.3fx)" % (ts, ts / em.out["value"]))
+
     if args.compare and args.params == "learned":
         # the same pencil with the PIPARMQ defaults, in a worker thread (ctypes releases the GIL); the
         # main thread waits until the budget and otherwise reports a lower bound

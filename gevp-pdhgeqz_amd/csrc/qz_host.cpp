@@ -984,9 +984,10 @@ int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi, dou
         ensure_ws(arena_bytes(n, prm));
         L.ws.top = 0;
         if (L.trace_cap > 0) L.trace.clear();
-        // TTOL = ulp * ||T(ilo:ihi, ilo:ihi)||_F (PDHGEQZ.f:456-458)
+        // TTOL = ulp * ||T(1:NH, 1:NH)||_F, NH = IHI - ILO + 1: the reference takes the leading NH x NH
+        // block (PDLANGE('F', NH, NH, T, 1, 1, ...), PDHGEQZ.f:453-458), not T(ILO:IHI, ILO:IHI)
         HIPCHK(hipMemsetAsync(L.d_norm, 0, sizeof(double), L.st));
-        gqz_launch_sumsq(L.st, T + (size_t)(ilo - 1) * (ldt + 1), ldt, ihi - ilo + 1, ihi - ilo + 1, 1, L.d_norm);
+        gqz_launch_sumsq(L.st, T, ldt, ihi - ilo + 1, ihi - ilo + 1, 1, L.d_norm);
         double tf2 = 0.0;
         sync_read(&tf2, L.d_norm, sizeof(double));
         R.ttol = std::max(kSafmin, kUlp * std::sqrt(tf2));

@@ -67,7 +67,10 @@ const char *gqz_last_error(void);
  * job: 'E' or 'S'; compq, compz: 'N', 'I', 'V' (PDHGEQZ.f:132-161).  ilo, ihi are 1-based.
  * On exit H = Q^T H_in Z (Schur form), T likewise, alphar/alphai/beta per PDHGEQZ.f:51-66
  * (beta >= 0, beta = 0 exactly for infinite eigenvalues).  Returns INFO (PDHGEQZ.f:496-502):
- * < 0 argument error, > 0 no convergence (index of the unconverged bottom), 0 success. */
+ * < 0 argument error, > 0 no convergence (index of the unconverged bottom), 0 success.
+ * The device entry point takes one leading dimension for H and T: ldt must equal ldh (INFO = -8
+ * otherwise; PDHGEQZ's DESCH and DESCT are separate).  The host entry points (gqz_hgeqz,
+ * gqz_pdhgeqz_host) accept any ldh, ldt, ldq, ldz: they copy into device arrays of their own. */
 int gqz_hgeqz_dev(char job, char compq, char compz, int n, int ilo, int ihi,
                   double *H, int ldh, double *T, int ldt,
                   double *alphar, double *alphai, double *beta,

@@ -1573,10 +1573,11 @@ int orc_hgeqz(int job, int compq, int compz, int n, int ilo, int ihi,
     c.Q = compq ? Q : NULL; c.Z = compz ? Z : NULL; c.ldq = ldq; c.ldz = ldz;
     if (compq == 1) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) Q[IX(i, j, ldq)] = (i == j);
     if (compz == 1) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) Z[IX(i, j, ldz)] = (i == j);
-    /* COMMON /PREC/ (PDHGEQZ.f:449-458) */
+    /* COMMON /PREC/ (PDHGEQZ.f:449-458): TFNORM is PDLANGE('F', NH, NH, T, 1, 1, ...), the leading
+     * NH x NH block of T (not the active block T(ILO:IHI, ILO:IHI) when ILO > 1) */
     int nh = ihi - ilo + 1;
     double tf = 0.0;
-    for (int j = ilo; j <= ihi; ++j) for (int i = ilo; i <= MINI(j, ihi); ++i) tf += T[IX(i, j, ldt)] * T[IX(i, j, ldt)];
+    for (int j = 0; j < nh; ++j) for (int i = 0; i <= j; ++i) tf += T[IX(i, j, ldt)] * T[IX(i, j, ldt)];
     c.smlnum = SAFMIN * ((double)nh / ULP);
     c.ttol = fmax(SAFMIN, ULP * sqrt(tf));
     c.alphar = alphar; c.alphai = alphai; c.beta = beta; c.st = st;
