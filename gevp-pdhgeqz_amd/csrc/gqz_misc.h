@@ -47,6 +47,20 @@ void gqz_launch_tgord_window(hipStream_t st, double *S, double *T, int ld, int p
 void gqz_launch_aed_spike(hipStream_t st, double *Hw, double *Tw, double *U, double *V, int jw,
                           const double *sptr, double *scr, const int *out);
 
+// Raise a kernel's dynamic-LDS limit on the current device to at least `bytes`, never lowering it:
+// the attribute is process-wide, so callers on several host threads go through one lock.
+#include <mutex>
+inline void gqz_need_lds(const void *fn, size_t bytes, size_t *cur_per_device) {
+    static std::mutex mu;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    std::lock_guard<std::mutex> g(mu);
+    if (bytes > cur_per_device[dev]) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        cur_per_device[dev] = bytes;
+    }
+}
+
 // library state shared with entry points living beside their kernels (qz_host.cpp)
 int gqz_init_current();                 // initialise on device 0 if needed and make it current; 0 = ok
 int gqz_ready_device();                 // the initialised device made current, -1 before gqz_init (no init)

@@ -693,11 +693,8 @@ void gqz_launch_aed_small(hipStream_t st, double *H, double *T, int ldh, int kwt
                           double ttol, double smlnum, double *U, double *V, double *war, double *wai, double *wb,
                           double *pscr, double *scr, int *out) {
     const size_t lds = gqz_aed_small_lds(jw);
-    static size_t lds_set = 0;
-    if (lds > lds_set) {
-        hipFuncSetAttribute((const void *)k_aed_small, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    static size_t lds_set[64] = {};
+    gqz_need_lds((const void *)k_aed_small, lds, lds_set);
     hipLaunchKernelGGL(k_aed_small, dim3(1), dim3(AEDS_NT), lds, st, H, T, ldh, kwtop, jw, has_spike, ttol, smlnum,
                        U, V, war, wai, wb, pscr, scr, out);
 }

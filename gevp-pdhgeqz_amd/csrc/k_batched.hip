@@ -175,7 +175,8 @@ struct Batch {
     std::vector<long> key;
     // tuned launch shape (0 = untuned)
     int tuned_nt = 0, tuned_nwg = 0;
-} B;
+};
+thread_local Batch B;             // per host thread (its buffers and graph belong to its stream)
 
 void launch(hipStream_t st, const BatchArgs &a, int nt, int nwg) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.counter);

@@ -20,6 +20,7 @@
 // with U (rows to the right) and V (columns above, Q, Z).
 #include "gqz_dev.h"
 #include "gqz_kernels.h"
+#include "gqz_misc.h"
 
 #define CH_MAXW 64
 #define CH_MAXB 21
@@ -266,11 +267,8 @@ void gqz_chase_set(hipStream_t st, const GqzChainSet &s) {
     int wm = 0;
     for (int c = 0; c < s.nch; ++c) wm = max(wm, s.w[c]);
     size_t lds = chase_lds(wm);
-    static size_t lds_set = 0;
-    if (lds > lds_set) {
-        hipFuncSetAttribute((const void *)k_chase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    static size_t lds_set[64] = {};
+    gqz_need_lds((const void *)k_chase, lds, lds_set);
     hipLaunchKernelGGL(k_chase, dim3(s.nch), dim3(CH_NT), lds, st, s);
 }
 

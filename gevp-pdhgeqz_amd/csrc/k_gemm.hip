@@ -15,6 +15,7 @@
 //      16x16x4 form uses D[4 * (l / 16) + r] instead).
 #include <hip/hip_runtime.h>
 #include "gqz_kernels.h"
+#include "gqz_misc.h"
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -304,11 +305,8 @@ static void launch_right(hipStream_t st, GqzSlabJobs &J, int kmax) {
     J.blk0[J.njobs] = tot;
     if (tot == 0) return;
     size_t lds = sizeof(double) * RB * (size_t)kmax;
-    static size_t lds_set = 0;
-    if (lds > lds_set) {
-        hipFuncSetAttribute((const void *)k_slab_right<RB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    static size_t lds_set[64] = {};
+    gqz_need_lds((const void *)k_slab_right<RB>, lds, lds_set);
     hipLaunchKernelGGL(k_slab_right<RB>, dim3(tot), dim3(256), lds, st, J);
 }
 template <int CB>
@@ -318,11 +316,8 @@ static void launch_left(hipStream_t st, GqzSlabJobs &J, int kmax) {
     J.blk0[J.njobs] = tot;
     if (tot == 0) return;
     size_t lds = sizeof(double) * (size_t)(kmax + 1) * CB;
-    static size_t lds_set = 0;
-    if (lds > lds_set) {
-        hipFuncSetAttribute((const void *)k_slab_left<CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    static size_t lds_set[64] = {};
+    gqz_need_lds((const void *)k_slab_left<CB>, lds, lds_set);
     hipLaunchKernelGGL(k_slab_left<CB>, dim3(tot), dim3(256), lds, st, J, kmax);
 }
 

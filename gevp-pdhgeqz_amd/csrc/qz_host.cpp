@@ -96,7 +96,11 @@ struct Lib {
     double progress_s = 0.0;         // > 0: top-level progress lines on stderr at this interval
     std::string err;
     std::mutex mu;
-} L;
+};
+// One context per host thread: its own stream, workspace arena, pinned staging and settings, so
+// calls from different host threads run concurrently (each thread's first call initialises its
+// context on device 0 unless it called gqz_init; gqz_finalize releases the calling thread's).
+thread_local Lib L;
 
 double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -158,7 +162,8 @@ struct KTimer {
     unsigned long long rng = 1;
     long timed[2 * GQZ_KC_COUNT] = {};
     int level = 0;
-} KT;
+};
+thread_local KTimer KT;           // per host thread, like the stream it times
 
 struct KtLevel {            // scoped recursion level for the timing classes
     int saved;
