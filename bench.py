@@ -420,6 +420,7 @@ def main():
                                                "PIPARMQ's"}
         log("static NW=48: %.2f s (learned gain %.3fx)" % (ts, ts / em.out["value"]))
 
+    abandon = False
     if args.compare and args.params == "learned":
         # the same pencil with the PIPARMQ defaults, in a worker thread (ctypes releases the GIL); the
         # main thread waits until the budget and otherwise reports a lower bound
@@ -454,6 +455,7 @@ def main():
             cmp_["note"] = "same pencil, same call (JOB=S, COMPQ=COMPZ=I), PIPARMQ default parameters"
             log("default: %.2f s (gain %.2fx)" % (res_box["t"], cmp_["gain"]))
         else:
+            abandon = True
             lb = time.perf_counter() - t_d0
             cmp_["default_lower_bound_s"] = lb
             cmp_["gain_lower_bound"] = lb / learned_s
@@ -462,8 +464,9 @@ def main():
             log(cmp_["note"])
     em.emit()
     sys.stdout.flush()
-    # a default run still in flight is abandoned with the process
-    os._exit(0)
+    if abandon:
+        # a default run still in flight is abandoned with the process
+        os._exit(0)
 
 
 if __name__ == "__main__":
