@@ -52,6 +52,9 @@ void gqz_slab_left(hipStream_t st, double *C, int ldc, int k, int n, const doubl
 // general out-of-place C = op(A) * op(B) (m x n, inner k); ta/tb transpose flags
 void gqz_gemm(hipStream_t st, int ta, int tb, int m, int n, int k, const double *A, int lda,
               const double *B, int ldb, double *C, int ldc);
+// C = alpha op(A) op(B) + beta C (beta = 0: C is only written)
+void gqz_gemm_ab(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc);
 
 // chain chase inside an LDS window (see k_chase.hip)
 struct GqzChainArgs {

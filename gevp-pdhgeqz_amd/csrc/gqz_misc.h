@@ -24,6 +24,10 @@ void gqz_launch_gghrd_blk(hipStream_t st, double *A, double *B, int ld, int n, i
                           double *Z, int ldq, int nq, void *ws);
 void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
                          double *tau);
+// blocked QR of B (compact WY panels of 32 columns, GEMM updates); ws: gqz_house_qr_ws_doubles(n)
+size_t gqz_house_qr_ws_doubles(int n);
+void gqz_launch_house_qr_blk(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
+                             double *tau, double *ws);
 void gqz_launch_sumsq(hipStream_t st, const double *A, int ld, int m, int n, int upper, double *out);
 void gqz_launch_sub(hipStream_t st, double *R, int ldr, const double *S, int lds, int n);
 // AED window post-processing (k_aed.hip); pscr holds (jw/2 + 1) * 65 doubles

@@ -354,11 +354,12 @@ void gqz_slab_left(hipStream_t st, double *C, int ldc, int k, int n, const doubl
     gqz_slab_left_multi(st, J);
 }
 
-// ---- general out-of-place GEMM: C = op(A) op(B), 64x64 tiles, k-steps of 16 via LDS ---------
+// ---- general GEMM: C = alpha op(A) op(B) + beta C (beta = 0: C not read), 64x64 tiles, k-steps
+// of 16 via LDS ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_gemm64(int ta, int tb, int m, int n, int k,
                                                 const double *__restrict__ A, int lda,
                                                 const double *__restrict__ B, int ldb,
-                                                double *__restrict__ C, int ldc) {
+                                                double *__restrict__ C, int ldc, double alpha, double beta) {
     __shared__ double sA[64 * 17];   // k-major: 16 k-rows of 64 rows (entries 1024.. unused)
     __shared__ double sB[16 * 65];   // 16 k x 64 cols
     const int tid = threadIdx.x;
@@ -400,14 +401,21 @@ __global__ void __launch_bounds__(256) k_gemm64(int ta, int tb, int m, int n, in
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 long row = i0 + wr + 16 * a + lk + 4 * r;
-                if (row < m) C[row + col * ldc] = acc[a][b][r];
+                if (row < m) {
+                    double *c = C + row + col * ldc;
+                    *c = (beta == 0.0) ? alpha * acc[a][b][r] : fma(alpha, acc[a][b][r], beta * *c);
+                }
             }
         }
 }
 
 void gqz_gemm(hipStream_t st, int ta, int tb, int m, int n, int k, const double *A, int lda,
               const double *B, int ldb, double *C, int ldc) {
+    gqz_gemm_ab(st, ta, tb, m, n, k, 1.0, A, lda, B, ldb, 0.0, C, ldc);
+}
+void gqz_gemm_ab(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc) {
     if (m <= 0 || n <= 0) return;
     dim3 grid((n + 63) / 64, (m + 63) / 64);
-    hipLaunchKernelGGL(k_gemm64, grid, dim3(256), 0, st, ta, tb, m, n, k, A, lda, B, ldb, C, ldc);
+    hipLaunchKernelGGL(k_gemm64, grid, dim3(256), 0, st, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, alpha, beta);
 }

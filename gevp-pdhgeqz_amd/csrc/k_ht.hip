@@ -1,9 +1,13 @@
 // k_ht.hip -- Householder QR of B applied to A and Q (the pre-step of the PDGGHRD drivers,
 // testing/EXRAND2.f:296-329: PDGEQRF + PDORMQR), plus norms for PCOMPRES-style checks.
 //
-// One reflector per column: a single workgroup forms it (k_house_make), then the workgroups of
-// k_house_left apply it to column panels of B (trailing) and A (all columns), and those of
-// k_house_right to row panels of Q.  Memory bound: each application streams the affected rows.
+// Blocked (gqz_launch_house_qr_blk): per panel of QR_NB columns the reflectors are formed one column
+// at a time (k_house_make, one workgroup) and applied inside the panel only (k_house_left); the panel's
+// product H_k ... H_{k+nb-1} = I - V T V^T (compact WY, T by DLARFT's forward recurrence in
+// k_qr_tmat) then goes to the trailing columns of B, to A and to Q as FP64-MFMA GEMMs (k_gemm64):
+// B <- B - V (T^T (V^T B)), A likewise, Q <- Q - ((Q V) T) V^T.  The unblocked form (one reflector per
+// column applied everywhere, memory bound) remains for small orders.
+#include <algorithm>
 #include "gqz_dev.h"
 #include "gqz_misc.h"
 
@@ -114,6 +118,81 @@ extern "C" __global__ void k_sub(double *R, int ldr, const double *S, int lds, i
         int i = (int)(idx % n), j = (int)(idx / n);
         R[GQZ_IX(i, j, ldr)] -= S ? S[GQZ_IX(i, j, lds)] : (i == j ? 1.0 : 0.0);
     }
+}
+
+// V (m x nbk, ld m): the panel's reflectors as a unit lower-trapezoidal matrix from B(k.., k..)
+extern "C" __global__ void k_qr_vcopy(const double *B, int ldb, int k, int m, int nbk, double *V) {
+    const long total = (long)m * nbk;
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % m), c = (int)(idx / m);
+        V[idx] = (i == c) ? 1.0 : (i > c ? B[GQZ_IX(k + i, k + c, ldb)] : 0.0);
+    }
+}
+
+// T (nbk x nbk upper triangular, ld nbk) with H_0 ... H_{nbk-1} = I - V T V^T, from G = V^T V and tau
+// (DLARFT, DIRECT = 'F', STOREV = 'C'): T(i,i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i)
+#define QR_NB 32
+extern "C" __global__ void __launch_bounds__(64) k_qr_tmat(const double *G, int nbk, const double *tau, double *T) {
+    __shared__ double sT[QR_NB * QR_NB], sG[QR_NB * QR_NB];
+    const int r = threadIdx.x;
+    for (int idx = r; idx < nbk * nbk; idx += blockDim.x) { sG[idx] = G[idx]; sT[idx] = 0.0; }
+    __syncthreads();
+    for (int i = 0; i < nbk; ++i) {
+        const double ti = tau[i];
+        double t = 0.0;
+        if (r < i) {
+            for (int c = r; c < i; ++c) t = fma(sT[r + nbk * c], sG[c + nbk * i], t);
+            t *= -ti;
+        }
+        __syncthreads();
+        if (r < i) sT[r + nbk * i] = t;
+        if (r == i) sT[i + nbk * i] = ti;
+        __syncthreads();
+    }
+    for (int idx = r; idx < nbk * nbk; idx += blockDim.x) T[idx] = sT[idx];
+}
+
+size_t gqz_house_qr_ws_doubles(int n) { return 5 * (size_t)n * QR_NB + 2 * (size_t)QR_NB * QR_NB + 256; }
+
+void gqz_launch_house_qr_blk(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
+                             double *tau, double *ws) {
+    double *V = ws, *W = V + (size_t)n * QR_NB, *W2 = W + (size_t)n * QR_NB, *Y = W2 + (size_t)n * QR_NB;
+    double *Y2 = Y + (size_t)n * QR_NB, *G = Y2 + (size_t)n * QR_NB, *T = G + QR_NB * QR_NB;
+    for (int k = 0; k < n - 1; k += QR_NB) {
+        const int nbk = std::min(QR_NB, n - 1 - k), m = n - k;
+        for (int j = k; j < k + nbk; ++j) {
+            hipLaunchKernelGGL(k_house_make, dim3(1), dim3(1024), 0, st, B, ldb, n, j, tau);
+            const int nc = k + nbk - (j + 1);
+            if (nc > 0)
+                hipLaunchKernelGGL(k_house_left, dim3((nc + 3) / 4), dim3(256), 0, st, B, ldb, n, j, tau, B, ldb,
+                                   j + 1, k + nbk);
+        }
+        {
+            const long tot = (long)m * nbk;
+            int g = (int)((tot + 255) / 256); if (g > 4096) g = 4096;
+            hipLaunchKernelGGL(k_qr_vcopy, dim3(g), dim3(256), 0, st, B, ldb, k, m, nbk, V);
+        }
+        gqz_gemm(st, 1, 0, nbk, nbk, m, V, m, V, m, G, nbk);
+        hipLaunchKernelGGL(k_qr_tmat, dim3(1), dim3(64), 0, st, G, nbk, tau + k, T);
+        // trailing columns of B, then all columns of A: C <- C - V (T^T (V^T C)), rows k..n-1
+        const int nct = n - (k + nbk);
+        double *Cs[2] = { B + GQZ_IX(k, k + nbk, ldb), A + GQZ_IX(k, 0, lda) };
+        const int ncs[2] = { nct, n }, lds[2] = { ldb, lda };
+        for (int q = 0; q < 2; ++q) {
+            if (ncs[q] <= 0) continue;
+            gqz_gemm(st, 1, 0, nbk, ncs[q], m, V, m, Cs[q], lds[q], W, nbk);
+            gqz_gemm(st, 1, 0, nbk, ncs[q], nbk, T, nbk, W, nbk, W2, nbk);
+            gqz_gemm_ab(st, 0, 0, m, ncs[q], nbk, -1.0, V, m, W2, nbk, 1.0, Cs[q], lds[q]);
+        }
+        // Q(:, k..n-1) <- Q(:, k..n-1) - ((Q V) T) V^T
+        if (Q) {
+            double *C = Q + GQZ_IX(0, k, ldq);
+            gqz_gemm(st, 0, 0, n, nbk, m, C, ldq, V, m, Y, n);
+            gqz_gemm(st, 0, 0, n, nbk, nbk, Y, n, T, nbk, Y2, n);
+            gqz_gemm_ab(st, 0, 1, n, m, nbk, -1.0, Y2, n, V, m, 1.0, C, ldq);
+        }
+    }
+    hipLaunchKernelGGL(k_zero_lower, dim3(n), dim3(256), 0, st, B, ldb, n);
 }
 
 void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,

@@ -1030,7 +1030,14 @@ int gqz_geqrb_dev(char compq, int n, double *A, int lda, double *B, int ldb, dou
         DevBuf tau_buf;
         double *tau = tau_buf.alloc_async(sizeof(double) * (size_t)n, L.st);
         if (icq == 1) gqz_launch_identity(L.st, Q, ldq, n, n);
-        gqz_launch_house_qr(L.st, n, A, lda, B, ldb, icq ? Q : nullptr, ldq, tau);
+        if (n > 64) {
+            DevBuf wbuf;
+            double *wq = wbuf.alloc_async(sizeof(double) * gqz_house_qr_ws_doubles(n), L.st);
+            gqz_launch_house_qr_blk(L.st, n, A, lda, B, ldb, icq ? Q : nullptr, ldq, tau, wq);
+            HIPCHK(hipStreamSynchronize(L.st));
+        } else {
+            gqz_launch_house_qr(L.st, n, A, lda, B, ldb, icq ? Q : nullptr, ldq, tau);
+        }
         HIPCHK(hipStreamSynchronize(L.st));
         return 0;
     } catch (const std::exception &e) {
