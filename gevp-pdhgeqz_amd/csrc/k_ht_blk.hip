@@ -214,11 +214,13 @@ extern "C" __global__ void __launch_bounds__(HT_NT) k_ht_lead(double *A, double 
             double b11 = sB[(w - 1) * (HT_R + 1)];
             double b01 = rdlane(cv, w - 2);
             double rc = rcv[w - 1], rs = rsv[w - 1], b00 = sB[(w - 2) * (HT_R + 1)];
+            double kcc = 0.0, kss = 0.0, kr2 = 0.0;                 // lane k: G and B(k, k) of position k
             for (int q = w - 1; q >= 1; --q) {
-                double xq = 0.0, yq = 0.0;
-                if (q >= 2) { xq = rdlane(cv, q - 2); yq = rdlane(pf, q - 2); }
-                double rcn = 0.0, rsn = 0.0, b00n = 0.0;
-                if (q >= 2) { rcn = rcv[q - 1]; rsn = rsv[q - 1]; b00n = sB[(q - 2) * (HT_R + 1)]; }
+                double xq = 0.0, yq = 0.0, rcn = 0.0, rsn = 0.0, b00n = 0.0;
+                if (q >= 2) {
+                    xq = rdlane(cv, q - 2); yq = rdlane(pf, q - 2);
+                    rcn = rcv[q - 1]; rsn = rsv[q - 1]; b00n = sB[(q - 2) * (HT_R + 1)];
+                }
                 const double nb00 = rc * b00, nb10 = -rs * b00;
                 const double nb01 = rc * b01 + rs * b11, nb11 = rc * b11 - rs * b01;
                 double cc, ss, r2;
@@ -226,47 +228,58 @@ extern "C" __global__ void __launch_bounds__(HT_NT) k_ht_lead(double *A, double 
                 const double n01 = cc * nb01 + ss * nb00;           // B(q-1, q)
                 b11 = cc * nb00 - ss * nb01;                        // B(q-1, q-1): the next corner's b11
                 b01 = cc * yq - ss * xq;                            // B(q-2, q-1): the next corner's b01
-                if (lane == 0) {
-                    sB[q * (HT_R + 1)] = r2;
-                    sB[q + HT_R * (q - 1)] = 0.0;
-                    HtRot t; t.rc = rc; t.rs = rs; t.gc = cc; t.gs = ss;
-                    srot[q] = t;
-                }
+                // the lane's pair: a row lane R_c on (B(q-1, k), B(q, k)), a column lane G_c on
+                // (B(k, q), B(k, q-1)); branch-free, one store
                 const bool rl = k >= q + 1 && k < w, gl = k <= q - 2;
-                if (rl || gl) {
-                    const double ca = rl ? rc : cc, sa = rl ? rs : ss;
-                    const double a = rl ? pf : cv, bb = rl ? cv : pf;
-                    const double u = ca * a + sa * bb, v = ca * bb - sa * a;
-                    if (rl) { sB[q + HT_R * k] = v; cv = u; }      // row q final; row q-1 carried
-                    else { sB[k + HT_R * q] = u; cv = v; }         // column q written; column q-1 carried
-                }
-                if (k == q) cv = n01;                              // lane q: row lane from position q-1
-                if (q >= 2) {
-                    if (k >= q && k < w) pf = sB[(q - 2) + HT_R * k];
-                    else if (k <= q - 3) pf = sB[k + HT_R * (q - 2)];
-                }
+                const double ca = rl ? rc : cc, sa = rl ? rs : ss;
+                const double a = rl ? pf : cv, bb = rl ? cv : pf;
+                const double u = ca * a + sa * bb, v = ca * bb - sa * a;
+                if (rl || gl) sB[rl ? q + HT_R * k : k + HT_R * q] = rl ? v : u;   // row q / column q final
+                cv = rl ? u : (gl ? v : (k == q ? n01 : cv));      // carried: row q-1 / column q-1
+                if (k == q) { kcc = cc; kss = ss; kr2 = r2; }
+                const bool pr = k >= q && k < w;
+                if (q >= 2 && (pr || k <= q - 3)) pf = sB[pr ? (q - 2) + HT_R * k : k + HT_R * (q - 2)];
                 rc = rcn; rs = rsn; b00 = b00n;
                 wave_sync();
             }
-            if (k >= 1 && k < w) sB[HT_R * k] = cv;                // row 0
-            if (lane == 0) {
-                sB[0] = b11;
-                // w_b and s_b = prod(-gs), prod(rs); then publish the block
-                double pg = 1.0, pr = 1.0;
-                for (int t = 0; t < w - 1; ++t) {
-                    const HtRot rt = srot[t + 1];
-                    rot[r0 + t + 1 - j - 2] = rt;
-                    const double wv = rt.gc * pg;
-                    sW[t] = wv;
-                    Wg[b * HT_R + t] = wv;
-                    pg *= -rt.gs;
-                    pr *= rt.rs;
+            if (k >= 1 && k < w) {
+                sB[HT_R * k] = cv;                                  // row 0
+                sB[k * (HT_R + 1)] = kr2;
+                sB[k + HT_R * (k - 1)] = 0.0;
+                HtRot t; t.rc = rcv[k]; t.rs = rsv[k]; t.gc = kcc; t.gs = kss;
+                srot[k] = t;
+            }
+            wave_sync();
+            {
+                // w_b(t) = gc_{t+1} prod_{m <= t} (-gs_m) and s_b = prod(-gs), prod(rs) by a scan across
+                // the lanes (lane t: position r0 + t + 1); then publish the block
+                double gcv = 0.0, ngs = 1.0, rsk = 1.0;
+                if (k < w - 1) {
+                    const HtRot rt = srot[k + 1];
+                    rot[r0 + k + 1 - j - 2] = rt;
+                    gcv = rt.gc; ngs = -rt.gs; rsk = rt.rs;
                 }
-                for (int t = w - 1; t < HT_K; ++t) sW[t] = 0.0;
-                sW[HT_K] = pg;
-                dGg[b] = pg;
-                dRg[b] = pr;
-                ht_post(&ctr[0], tag | (unsigned long long)(b + 1));
+                double incl = ngs;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const double o = __shfl_up(incl, off);
+                    if (k >= off) incl *= o;
+                }
+                double excl = __shfl_up(incl, 1);
+                if (k == 0) excl = 1.0;
+                const double wv = gcv * excl;
+                if (k < w - 1) { sW[k] = wv; Wg[b * HT_R + k] = wv; }
+                else if (k < HT_K) sW[k] = 0.0;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) rsk *= __shfl_xor(rsk, off);
+                const double pg = rdlane(incl, w - 2);
+                if (lane == 0) {
+                    sB[0] = b11;
+                    sW[HT_K] = pg;
+                    dGg[b] = pg;
+                    dRg[b] = rsk;
+                    ht_post(&ctr[0], tag | (unsigned long long)(b + 1));
+                }
             }
         } else if (b + 1 < nb) {
             // region b+1 (rows/columns r0n .. r0): its entries are untouched so far, except the last
