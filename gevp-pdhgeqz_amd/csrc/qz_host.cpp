@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -80,6 +81,10 @@ struct Arena {
 struct Lib {
     int dev = -1;
     hipStream_t st = nullptr;
+    // second stream (default priority) for the slab updates of a sweep, overlapped with the chase
+    hipStream_t st2 = nullptr;
+    bool overlap = true;
+    std::vector<hipEvent_t> sev;     // ordering events between the two streams (no timing)
     Arena ws;
     int *d_out = nullptr, *h_out = nullptr;
     double *h_buf = nullptr;   // pinned staging (readbacks and shift uploads)
@@ -149,7 +154,7 @@ struct Run {
 // ---- per-class kernel timing: HIP events on the library stream around each launch --------------
 struct KTimer {
     unsigned mask = 0;
-    struct Pend { int cls; hipEvent_t e0, e1; double flops, bytes; };
+    struct Pend { int cls; hipEvent_t e0, e1; double flops, bytes; hipStream_t st; };
     std::vector<Pend> pend;
     std::vector<hipEvent_t> pool;
     // classes 0..GQZ_KC_COUNT-1: top level; +GQZ_KC_COUNT: inside AED windows (nested levels)
@@ -180,15 +185,15 @@ hipEvent_t kt_event() {
 
 // returns a slot (or -1 when the class is not timed); the matching kt_end records the stop event.
 // Slots <= -2 stand for an untimed launch of class -slot - 2: only its work is counted.
-int kt_begin(int cls) {
+int kt_begin(int cls, hipStream_t st = nullptr) {
     if (!(KT.mask & (1u << cls))) return -1;
     const int c = cls + (KT.level > 0 ? GQZ_KC_COUNT : 0);
     if (KT.every > 1) {
         KT.rng = KT.rng * 6364136223846793005ULL + 1442695040888963407ULL;
         if ((KT.rng >> 33) % (unsigned long long)KT.every != 0) return -2 - c;
     }
-    KTimer::Pend p = { c, kt_event(), kt_event(), 0.0, 0.0 };
-    HIPCHK(hipEventRecord(p.e0, L.st));
+    KTimer::Pend p = { c, kt_event(), kt_event(), 0.0, 0.0, st ? st : L.st };
+    HIPCHK(hipEventRecord(p.e0, p.st));
     KT.pend.push_back(p);
     return (int)KT.pend.size() - 1;
 }
@@ -202,7 +207,7 @@ void kt_end(int slot, double flops, double bytes) {
     }
     KTimer::Pend &p = KT.pend[slot];
     p.flops = flops; p.bytes = bytes;
-    HIPCHK(hipEventRecord(p.e1, L.st));
+    HIPCHK(hipEventRecord(p.e1, p.st));
 }
 
 // fold completed launches into the totals (the stream has been synchronised)
@@ -241,13 +246,14 @@ void slab_work(const GqzSlabJobs &J, double &flops, double &bytes) {
 // where the two products commute.
 struct Win { int p, w; const double *U, *V; };
 
-void slab_launch(Run &R, GqzSlabJobs &J, bool left) {
+void slab_launch(Run &R, GqzSlabJobs &J, bool left, hipStream_t st = nullptr) {
     if (J.njobs == 0) return;
+    if (!st) st = L.st;
     double fl, by;
     slab_work(J, fl, by);
-    int ks = kt_begin(GQZ_KC_SLAB);
-    if (left) gqz_slab_left_multi(L.st, J);
-    else gqz_slab_right_multi(L.st, J);
+    int ks = kt_begin(GQZ_KC_SLAB, st);
+    if (left) gqz_slab_left_multi(st, J);
+    else gqz_slab_right_multi(st, J);
     kt_end(ks, fl, by);
     R.st.slab_launches++;
     J.njobs = 0;
@@ -448,35 +454,70 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
 // disjoint.  Each chain sees exactly the window sequence of a lone chain (intro window at ktop,
 // then windows starting where the previous one stopped), so the result does not depend on how
 // many chains share a round.
-struct Chain { int soff, nb, gtop; bool started, done; };
+//
+// Overlap (L.overlap): the updates of round r run on the second stream while round r + 1 is
+// chased.  Only the columns that a chain's next window reads ("near" columns [p + w, p' + w'),
+// p' = p + steps) get their left update on the chase stream before the next chase; the rest of
+// the left update ("far") and the right update (rows above, Q, Z) follow on the second stream:
+//     stream 1: chase(r) -> wait B(r-1) -> left-near(r) -> event A(r) -> chase(r+1) -> ...
+//     stream 2: wait A(r) -> left-far(r) -> right(r) -> event B(r)
+// chase(r+1) touches nothing that B(r) touches when no window of round r+1 reaches into the
+// columns [p, p + w) of a round-r window of a chain ahead of it (right(r) writes rows above p of
+// those columns): the scheduler keeps every window above the round-r window start of each chain
+// ahead.  left-near(r+1) waits for B(r): both update rows of the same window (far columns of
+// round r, the orders of U(r) and U(r+1) matter) and rows of the window behind.  U, V of
+// consecutive rounds use alternate slots, since B(r) still reads round r's while round r+1 is
+// chased.
+struct Chain { int soff, nb, gtop; bool started, done; int prevp; };
+
+hipEvent_t sync_event(size_t i) {
+    while (L.sev.size() <= i) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        L.sev.push_back(e);
+    }
+    return L.sev[i];
+}
 
 void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> &ch, const double *d_sr,
                   const double *d_si, const double *d_sb) {
     const int WMAX = gqz_chase_max_window();
     const int nslot = GQZ_MAXCH;
+    const bool ovl = L.overlap && L.st2;
     size_t mk = L.ws.mark();
-    double *U = L.ws.alloc((size_t)nslot * WMAX * WMAX), *V = L.ws.alloc((size_t)nslot * WMAX * WMAX);
+    const size_t slot = (size_t)WMAX * WMAX;
+    double *U = L.ws.alloc(2 * (size_t)nslot * slot), *V = L.ws.alloc(2 * (size_t)nslot * slot);
     const int nch = (int)ch.size();
+    for (auto &x : ch) x.prevp = INT32_MAX;
     int first = 0;                       // oldest chain not yet done
+    long round = 0;
+    bool b_pending = false;              // stream 2 holds the updates of the previous round
     while (first < nch) {
+        const int par = (int)(round & 1);
+        double *Ur = U + (size_t)par * nslot * slot, *Vr = V + (size_t)par * nslot * slot;
         GqzChainSet S = {};
-        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = U; S.V = V; S.ustride = WMAX * WMAX;
+        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = Ur; S.V = Vr; S.ustride = WMAX * WMAX;
         S.sr = d_sr; S.si = d_si; S.sb = d_sb; S.smlnum = v.smlnum;
         Win ws[GQZ_MAXCH];
         int slot_chain[GQZ_MAXCH];
         int nw = 0;
-        // rows from which the chain ahead (already scheduled this round) forbids access
+        // rows from which the chains ahead forbid access
         int limit = kbot + 1;
         int active = 0;
-        for (int c = first; c < nch && nw < nslot; ++c) {
+        for (int c = 0; c < nch && nw < nslot; ++c) {
             Chain &x = ch[c];
-            if (x.done) continue;
+            const int pp = x.prevp;
+            if (x.done) {
+                if (ovl) limit = std::min(limit, pp);
+                continue;
+            }
             if (!x.started && active >= nslot) break;
             int p = x.started ? x.gtop : ktop;
             int w = std::min(WMAX, kbot - p + 1);
             if (p + w > limit) {
                 // wait: the chain ahead still occupies rows below; nothing behind may pass
-                limit = x.started ? x.gtop : ktop;
+                limit = std::min(limit, x.started ? x.gtop : ktop);
+                if (ovl) limit = std::min(limit, pp);
                 if (!x.started) break;
                 ++active;
                 continue;
@@ -486,28 +527,83 @@ void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> 
             S.p[q] = p; S.w[q] = w; S.nb[q] = x.nb; S.soff[q] = x.soff;
             S.intro[q] = x.started ? 0 : 1; S.t0[q] = x.started ? 0 : -1; S.last[q] = last;
             S.steps[q] = last ? w : (w - 3 * x.nb - 1);
-            ws[q] = Win{ p, w, U + (size_t)q * WMAX * WMAX, V + (size_t)q * WMAX * WMAX };
+            ws[q] = Win{ p, w, Ur + (size_t)q * slot, Vr + (size_t)q * slot };
             slot_chain[q] = c;
             limit = p;
+            if (ovl) limit = std::min(limit, pp);
             ++active;
         }
-        if (nw == 0) throw GqzError("chain scheduler made no progress");
+        if (nw == 0) {
+            bool pend = false;
+            for (auto &x : ch) pend = pend || x.prevp != INT32_MAX;
+            if (!ovl || !pend) throw GqzError("chain scheduler made no progress");
+            // the only obstacle is the previous round's right update: let it finish first
+            HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
+            for (auto &x : ch) x.prevp = INT32_MAX;
+            continue;
+        }
         S.nch = nw;
         double by = 0.0;
         for (int q = 0; q < nw; ++q) by += 48.0 * S.w[q] * S.w[q];
         int kc = kt_begin(GQZ_KC_CHASE);
         gqz_chase_set(L.st, S);
         kt_end(kc, 0.0, by);
-        apply_windows(R, v, ws, nw);
         R.st.chase_windows += nw;
+        for (auto &x : ch) x.prevp = INT32_MAX;
+        if (!ovl) {
+            apply_windows(R, v, ws, nw);
+        } else {
+            // near columns first, on the chase stream, after the previous round's far/right updates
+            if (b_pending) HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
+            GqzSlabJobs J;
+            J.njobs = 0;
+            int nearc[GQZ_MAXCH];
+            for (int q = 0; q < nw; ++q) {
+                const Win &x = ws[q];
+                nearc[q] = 0;
+                if (!S.last[q]) {
+                    const int np = x.p + S.steps[q], nw2 = std::min(WMAX, kbot - np + 1);
+                    nearc[q] = std::max(0, std::min(np + nw2, v.n) - (x.p + x.w));
+                }
+                if (nearc[q] <= 0) continue;
+                gqz_slab_add(J, v.H + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, nearc[q], x.U, x.w, x.w);
+                gqz_slab_add(J, v.T + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, nearc[q], x.U, x.w, x.w);
+            }
+            slab_launch(R, J, true);
+            HIPCHK(hipEventRecord(sync_event(0), L.st));
+            HIPCHK(hipStreamWaitEvent(L.st2, sync_event(0), 0));
+            for (int q = 0; q < nw; ++q) {
+                const Win &x = ws[q];
+                const int c0 = x.p + x.w + nearc[q], ncol = v.n - c0;
+                if (J.njobs + 2 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, true, L.st2);
+                gqz_slab_add(J, v.H + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
+                gqz_slab_add(J, v.T + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
+            }
+            slab_launch(R, J, true, L.st2);
+            for (int q = 0; q < nw; ++q) {
+                const Win &x = ws[q];
+                if (J.njobs + 4 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, false, L.st2);
+                gqz_slab_add(J, v.H + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+                gqz_slab_add(J, v.T + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+                if (v.Q) gqz_slab_add(J, v.Q + (size_t)x.p * v.ldq, v.ldq, v.nq, x.U, x.w, x.w);
+                if (v.Z) gqz_slab_add(J, v.Z + (size_t)x.p * v.ldq, v.ldq, v.nq, x.V, x.w, x.w);
+            }
+            slab_launch(R, J, false, L.st2);
+            HIPCHK(hipEventRecord(sync_event(1), L.st2));
+            b_pending = true;
+        }
         for (int q = 0; q < nw; ++q) {
             Chain &x = ch[slot_chain[q]];
             x.started = true;
             x.gtop = S.p[q] + S.steps[q];
+            x.prevp = S.p[q];
             if (S.last[q]) x.done = true;
         }
         while (first < nch && ch[first].done) ++first;
+        ++round;
     }
+    // everything after the sweep sees the updates of the last round
+    if (b_pending) HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
     L.ws.release(mk);
 }
 
@@ -578,7 +674,7 @@ void sweep(Run &R, const View &v, int ktop, int kbot, std::vector<double> &sh, i
         int cnt = std::min(chain, ns - off);
         cnt -= cnt % 2;
         if (cnt < 2) break;
-        chains.push_back(Chain{ off, cnt / 2, ktop, false, false });
+        chains.push_back(Chain{ off, cnt / 2, ktop, false, false, INT32_MAX });
     }
     chase_chains(R, v, ktop, kbot, chains, d_sh, d_sh + ns, d_sh + 2 * ns);
     // the staging buffer is reused by the next readback: make sure the upload has completed
@@ -804,6 +900,7 @@ int gqz_init(int device) {
         if (L.dev == device) return 0;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&L.st2, hipStreamNonBlocking));
         HIPCHK(hipMalloc((void **)&L.d_out, 64 * sizeof(int)));
         HIPCHK(hipMalloc((void **)&L.d_norm, 8 * sizeof(double)));
         HIPCHK(hipHostMalloc((void **)&L.h_out, 64 * sizeof(int), hipHostMallocDefault));
@@ -826,6 +923,7 @@ int gqz_finalize(void) {
     auto note = [&first](hipError_t e) { if (first == hipSuccess) first = e; };
     note(hipSetDevice(L.dev));
     note(hipStreamSynchronize(L.st));
+    if (L.st2) note(hipStreamSynchronize(L.st2));
     if (L.ws.base) note(hipFree(L.ws.base));
     L.ws = Arena();
     note(hipFree(L.d_out)); note(hipFree(L.d_norm));
@@ -837,8 +935,11 @@ int gqz_finalize(void) {
     L.d_out = nullptr; L.d_norm = nullptr; L.h_out = nullptr; L.h_buf = nullptr;
     L.d_mlp = L.d_mlp_x = L.d_mlp_y = L.h_mlp_io = nullptr;
     L.mlp_on = false;
+    for (hipEvent_t e : L.sev) note(hipEventDestroy(e));
+    L.sev.clear();
     note(hipStreamDestroy(L.st));
-    L.st = nullptr; L.dev = -1;
+    if (L.st2) note(hipStreamDestroy(L.st2));
+    L.st = nullptr; L.st2 = nullptr; L.dev = -1;
     if (first != hipSuccess) {
         L.err = std::string("gqz_finalize: ") + hipGetErrorString(first);
         return -1;
@@ -956,6 +1057,11 @@ int gqz_mlp_policy(const float *params_host, int enable) {
 int gqz_set_policy(gqz_policy_fn fn, void *user) { L.pol = fn; L.pol_user = user; return 0; }
 int gqz_trace_enable(int cap) { L.trace.clear(); L.trace_cap = std::max(0, cap); return 0; }
 int gqz_progress(double seconds) { L.progress_s = seconds > 0.0 ? seconds : 0.0; return 0; }
+int gqz_overlap(int on) {
+    const int prev = L.overlap ? 1 : 0;
+    if (on >= 0) L.overlap = on != 0;
+    return prev;
+}
 int gqz_trace_get(gqz_trace_rec *out, int cap) {
     int k = std::min<int>(cap, (int)L.trace.size());
     if (out) for (int i = 0; i < k; ++i) out[i] = L.trace[i];

@@ -6,8 +6,9 @@
  * on the library's HIP stream; the host-pointer entry points copy in, run, and copy out.
  * Indices in this header are 1-based where they mirror a Fortran argument (ILO, IHI) and
  * 0-based nowhere else.  No torch types cross this boundary.
- * Not re-entrant (like the reference, whose tolerances live in COMMON blocks): the stream, the
- * workspace arena and the pinned staging buffers are shared, so concurrent callers serialise.
+ * Re-entrant across host threads (the reference keeps its tolerances in COMMON blocks and is not):
+ * each host thread has its own context (streams, workspace arena, pinned staging, settings), created
+ * on device 0 by its first call unless it called gqz_init.  Calls from one thread are sequential.
  */
 #ifndef GQZ_H
 #define GQZ_H
@@ -60,6 +61,12 @@ int gqz_trace_get(gqz_trace_rec *out, int cap);   /* returns the number of recor
  * one line to stderr (top-level iteration, active block, sweeps and AED calls so far) at most every
  * `seconds`; 0 turns it off (the default). */
 int gqz_progress(double seconds);
+/* Sweep update overlap (no counterpart in the reference): with on = 1 (the default) the updates
+ * outside the chase windows of one round run on a second stream of the library (same priority)
+ * while the next round is chased; 0 runs them on the library stream after each chase (same
+ * products, in a different order).  on < 0 only queries.  Returns the previous setting; per host
+ * thread, like the rest of the context. */
+int gqz_overlap(int on);
 const char *gqz_last_error(void);
 
 /* ---- QZ ----------------------------------------------------------------------------------
