@@ -3,7 +3,8 @@
 //
 // Each workgroup repeatedly takes the next pencil from an atomic work counter (pencils are
 // ordered by decreasing order so the longest jobs start first) and runs the whole pipeline on
-// it: Householder QR of B applied to A (the EXRAND2.f:296-329 pre-step), Givens HT reduction
+// it: blocked Householder QR of B applied to A (the EXRAND2.f:296-329 pre-step, compact WY panels with
+// FP64-MFMA updates inside the workgroup, gqz_qrwg.h), Givens HT reduction
 // (PDGGHRD semantics) and the double-shift base QZ with infinite-eigenvalue deflation
 // (KKQZ/QZLAP role), accumulating Q and Z when requested.  Matrices stay in HBM (L2-resident
 // for the sizes of the batched configuration); every step is workgroup-cooperative.
@@ -13,6 +14,7 @@
 
 #include "gqz_ds.h"
 #include "gqz_misc.h"
+#include "gqz_qrwg.h"
 
 extern "C" void *gqz_stream(void);
 
@@ -27,63 +29,10 @@ struct BatchArgs {
     int *counter;
 };
 
-// Householder QR of B (n x n, ld) applied to A and accumulated into Q (Q = I on entry)
-__device__ void wg_qr_b(double *A, double *B, double *Q, int n, int ld, double *red, double *shv) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int j = 0; j < n - 1; ++j) {
-        double *col = B + GQZ_IX(0, j, ld);
-        double ss = 0.0;
-        for (int i = j + 1 + tid; i < n; i += nt) ss += col[i] * col[i];
-        red[tid] = ss;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) { if (tid < s) red[tid] += red[tid + s]; __syncthreads(); }
-        if (tid == 0) {
-            double xnorm = sqrt(red[0]);
-            double alpha = col[j];
-            if (xnorm == 0.0) { shv[0] = 0.0; shv[1] = 0.0; }
-            else {
-                double beta = -copysign(hypot(alpha, xnorm), alpha);
-                shv[0] = (beta - alpha) / beta;      // tau
-                shv[1] = 1.0 / (alpha - beta);       // scale of the tail
-                col[j] = beta;
-            }
-        }
-        __syncthreads();
-        const double tau = shv[0], sc = shv[1];
-        if (tau == 0.0) { __syncthreads(); continue; }
-        for (int i = j + 1 + tid; i < n; i += nt) col[i] *= sc;
-        __syncthreads();
-        // apply to B(:, j+1:n), A(:, 0:n) (columns) and Q(:, j:n) (rows)
-        const int nb_ = n - j - 1;
-        for (int idx = tid; idx < nb_ + n + (Q ? n : 0); idx += nt) {
-            if (idx < nb_ + n) {
-                double *m = (idx < nb_) ? B + GQZ_IX(0, j + 1 + idx, ld) : A + GQZ_IX(0, idx - nb_, ld);
-                double s = m[j];
-                for (int i = j + 1; i < n; ++i) s += col[i] * m[i];
-                s *= tau;
-                m[j] -= s;
-                for (int i = j + 1; i < n; ++i) m[i] -= s * col[i];
-            } else {
-                const int r = idx - nb_ - n;
-                double s = Q[GQZ_IX(r, j, ld)];
-                for (int i = j + 1; i < n; ++i) s += Q[GQZ_IX(r, i, ld)] * col[i];
-                s *= tau;
-                Q[GQZ_IX(r, j, ld)] -= s;
-                for (int i = j + 1; i < n; ++i) Q[GQZ_IX(r, i, ld)] -= s * col[i];
-            }
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < n * n; idx += nt) {
-        int i = idx % n, j = idx / n;
-        if (i > j) B[GQZ_IX(i, j, ld)] = 0.0;
-    }
-    __syncthreads();
-}
-
 template <int NT>
 __global__ void __launch_bounds__(NT) k_batched(BatchArgs a) {
     __shared__ WgCtl Cs;
+    __shared__ QrwLds qs;
     __shared__ double red[NT];
     __shared__ double sh[8];
     __shared__ int s_job;
@@ -108,7 +57,7 @@ __global__ void __launch_bounds__(NT) k_batched(BatchArgs a) {
             if (Z) Z[GQZ_IX(i, j, n)] = (i == j) ? 1.0 : 0.0;
         }
         __syncthreads();
-        wg_qr_b(A, B, Q, n, n, red, sh);
+        wg_qr_b_blk(A, B, Q, n, n, qs);
         wg_gghrd(A, B, n, n, 0, n - 1, Q, Z, n, n, sh);
         // TTOL = ulp * ||T||_F (PDHGEQZ.f:456-458)
         double ss = 0.0;
