@@ -46,7 +46,7 @@ static __device__ __forceinline__ double qrw_wave_sum(double x) {
 }
 
 // sum over the workgroup (every thread receives the total); red holds one slot per wave
-static __device__ double qrw_block_sum(double x, double *red) {
+static __device__ __forceinline__ double qrw_block_sum(double x, double *red) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     x = qrw_wave_sum(x);
     if (lane == 0) red[wave] = x;
@@ -65,7 +65,7 @@ static __device__ __forceinline__ double qrw_v(const double *B, int ld, int j0, 
 }
 
 // stage rows [r0, r0 + QRW_RC) of V into sV (zero outside the panel); caller synchronises
-static __device__ void qrw_stage(QrwLds &S, const double *B, int ld, int j0, int m, int nb, int r0) {
+static __device__ __forceinline__ void qrw_stage(QrwLds &S, const double *B, int ld, int j0, int m, int nb, int r0) {
     for (int idx = threadIdx.x; idx < QRW_RC * QRW_NB; idx += blockDim.x) {
         const int r = idx % QRW_RC, b = idx / QRW_RC;
         S.sV[r * QRW_LV + b] = qrw_v(B, ld, j0, m, nb, r0 + r, b);
@@ -74,42 +74,45 @@ static __device__ void qrw_stage(QrwLds &S, const double *B, int ld, int j0, int
 
 // C <- C - V T^T V^T C for the columns [c0, c0 + ncb) (ncb <= QRW_CB) of the m x . matrix C (rows
 // j0.. of the pencil matrix M, leading dimension ld)
-static __device__ void qrw_left_block(QrwLds &S, const double *B, double *M, int ld, int j0, int m, int nb, int c0,
+template <int NT>
+static __device__ __forceinline__ void qrw_left_block(QrwLds &S, const double *B, double *M, int ld, int j0, int m, int nb, int c0,
                                       int ncb) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     double *C = M + GQZ_IX(j0, c0, ld);
     // step 1: W^T = C^T V, items (column tile t, half h of V's columns)
-    qrw_d4 acc[8];
+    constexpr int NWV = NT / 64, NQ = (8 + NWV - 1) / NWV;     // waves, items per wave
+    qrw_d4 acc[NQ];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = qrw_d4{ 0.0, 0.0, 0.0, 0.0 };
+    for (int q = 0; q < NQ; ++q) acc[q] = qrw_d4{ 0.0, 0.0, 0.0, 0.0 };
     for (int r0 = 0; r0 < m; r0 += QRW_RC) {
         qrw_stage(S, B, ld, j0, m, nb, r0);
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int it = wave + q * nw;
-            if (it >= 8) break;
-            const int t = it >> 1, h = it & 1;
+        for (int q = 0; q < NQ; ++q) {
+            const int it = wave + q * NWV;
+            const int t = (it >> 1) & 3, h = it & 1;
             const int col = 16 * t + li;
-            if (16 * t >= ncb) continue;
+            if (it < 8 && 16 * t < ncb) {
 #pragma unroll 4
-            for (int s = 0; s < QRW_RC / 4; ++s) {
-                const int r = r0 + 4 * s + lk;
-                const double a = (r < m && col < ncb) ? C[GQZ_IX(r, col, ld)] : 0.0;
-                const double b = S.sV[(4 * s + lk) * QRW_LV + 16 * h + li];
-                acc[q] = qrw_mfma(a, b, acc[q]);
+                for (int s = 0; s < QRW_RC / 4; ++s) {
+                    const int r = r0 + 4 * s + lk;
+                    const double a = (r < m && col < ncb) ? C[GQZ_IX(r, col, ld)] : 0.0;
+                    const double b = S.sV[(4 * s + lk) * QRW_LV + 16 * h + li];
+                    acc[q] = qrw_mfma(a, b, acc[q]);
+                }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int it = wave + q * nw;
-        if (it >= 8) break;
-        const int t = it >> 1, h = it & 1;
+    for (int q = 0; q < NQ; ++q) {
+        const int it = wave + q * NWV;
+        const int t = (it >> 1) & 3, h = it & 1;
+        if (it < 8) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S.sW[(16 * h + li) * QRW_LW + 16 * t + lk + 4 * r] = acc[q][r];
+            for (int r = 0; r < 4; ++r) S.sW[(16 * h + li) * QRW_LW + 16 * t + lk + 4 * r] = acc[q][r];
+        }
     }
     __syncthreads();
     // step 2: W2 = -T^T W  (T^T lower triangular)
@@ -147,43 +150,46 @@ static __device__ void qrw_left_block(QrwLds &S, const double *B, double *M, int
 }
 
 // Q(q0 : q0 + nr, j0 : n) <- Q - ((Q V) T) V^T for one block of nr <= 64 rows of Q
-static __device__ void qrw_right_block(QrwLds &S, const double *B, double *Q, int ld, int j0, int m, int nb, int q0,
+template <int NT>
+static __device__ __forceinline__ void qrw_right_block(QrwLds &S, const double *B, double *Q, int ld, int j0, int m, int nb, int q0,
                                        int nr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     double *C = Q + GQZ_IX(q0, j0, ld);
     // step 1: Y = C V (nr x nb), items (row tile t, half h)
-    qrw_d4 acc[8];
+    constexpr int NWV = NT / 64, NQ = (8 + NWV - 1) / NWV;     // waves, items per wave
+    qrw_d4 acc[NQ];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = qrw_d4{ 0.0, 0.0, 0.0, 0.0 };
+    for (int q = 0; q < NQ; ++q) acc[q] = qrw_d4{ 0.0, 0.0, 0.0, 0.0 };
     for (int r0 = 0; r0 < m; r0 += QRW_RC) {
         qrw_stage(S, B, ld, j0, m, nb, r0);
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int it = wave + q * nw;
-            if (it >= 8) break;
-            const int t = it >> 1, h = it & 1;
+        for (int q = 0; q < NQ; ++q) {
+            const int it = wave + q * NWV;
+            const int t = (it >> 1) & 3, h = it & 1;
             const int row = 16 * t + li;
-            if (16 * t >= nr) continue;
+            if (it < 8 && 16 * t < nr) {
 #pragma unroll 4
-            for (int s = 0; s < QRW_RC / 4; ++s) {
-                const int k = r0 + 4 * s + lk;
-                const double a = (row < nr && k < m) ? C[GQZ_IX(row, k, ld)] : 0.0;
-                const double b = S.sV[(4 * s + lk) * QRW_LV + 16 * h + li];
-                acc[q] = qrw_mfma(a, b, acc[q]);
+                for (int s = 0; s < QRW_RC / 4; ++s) {
+                    const int k = r0 + 4 * s + lk;
+                    const double a = (row < nr && k < m) ? C[GQZ_IX(row, k, ld)] : 0.0;
+                    const double b = S.sV[(4 * s + lk) * QRW_LV + 16 * h + li];
+                    acc[q] = qrw_mfma(a, b, acc[q]);
+                }
             }
         }
         __syncthreads();
     }
     // D[lk + 4r][li] = Y[16 t + lk + 4 r][16 h + li]  ->  sW[b][row]
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int it = wave + q * nw;
-        if (it >= 8) break;
-        const int t = it >> 1, h = it & 1;
+    for (int q = 0; q < NQ; ++q) {
+        const int it = wave + q * NWV;
+        const int t = (it >> 1) & 3, h = it & 1;
+        if (it < 8) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S.sW[(16 * h + li) * QRW_LW + 16 * t + lk + 4 * r] = acc[q][r];
+            for (int r = 0; r < 4; ++r) S.sW[(16 * h + li) * QRW_LW + 16 * t + lk + 4 * r] = acc[q][r];
+        }
     }
     __syncthreads();
     // step 2: Y2 = -Y T  ->  sW2[a][row] = -sum_{b <= a} Y[row][b] T[b][a]
@@ -223,7 +229,8 @@ static __device__ void qrw_right_block(QrwLds &S, const double *B, double *Q, in
 // Householder QR of B (n x n, ld) applied to A (B <- Q^T B = R, A <- Q^T A) and accumulated into Q
 // (Q = I on entry; may be null).  Reflectors as LAPACK's DGEQR2/DLARFG: beta = -sign(alpha) ||x||,
 // tau = (beta - alpha) / beta, v = x / (alpha - beta) below the diagonal.
-static __device__ void wg_qr_b_blk(double *A, double *B, double *Q, int n, int ld, QrwLds &S) {
+template <int NT>
+static __device__ __forceinline__ void wg_qr_b_blk(double *A, double *B, double *Q, int n, int ld, QrwLds &S) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     for (int j0 = 0; j0 < n - 1; j0 += QRW_NB) {
@@ -315,10 +322,10 @@ static __device__ void wg_qr_b_blk(double *A, double *B, double *Q, int n, int l
         __syncthreads();
         // ---- trailing columns of B, all columns of A
         const int cB = j0 + nb, ncB = n - cB;
-        for (int c0 = 0; c0 < ncB; c0 += QRW_CB) qrw_left_block(S, B, B, ld, j0, m, nb, cB + c0, min(QRW_CB, ncB - c0));
-        for (int c0 = 0; c0 < n; c0 += QRW_CB) qrw_left_block(S, B, A, ld, j0, m, nb, c0, min(QRW_CB, n - c0));
+        for (int c0 = 0; c0 < ncB; c0 += QRW_CB) qrw_left_block<NT>(S, B, B, ld, j0, m, nb, cB + c0, min(QRW_CB, ncB - c0));
+        for (int c0 = 0; c0 < n; c0 += QRW_CB) qrw_left_block<NT>(S, B, A, ld, j0, m, nb, c0, min(QRW_CB, n - c0));
         if (Q)
-            for (int q0 = 0; q0 < n; q0 += 64) qrw_right_block(S, B, Q, ld, j0, m, nb, q0, min(64, n - q0));
+            for (int q0 = 0; q0 < n; q0 += 64) qrw_right_block<NT>(S, B, Q, ld, j0, m, nb, q0, min(64, n - q0));
     }
     for (int idx = tid; idx < n * n; idx += blockDim.x) {
         const int i = idx % n, j = idx / n;

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(NT) k_batched(BatchArgs a) {
             if (Z) Z[GQZ_IX(i, j, n)] = (i == j) ? 1.0 : 0.0;
         }
         __syncthreads();
-        wg_qr_b_blk(A, B, Q, n, n, qs);
+        wg_qr_b_blk<NT>(A, B, Q, n, n, qs);
         wg_gghrd(A, B, n, n, 0, n - 1, Q, Z, n, n, sh);
         // TTOL = ulp * ||T||_F (PDHGEQZ.f:456-458)
         double ss = 0.0;
