@@ -445,22 +445,24 @@ def main():
         log("cpu baseline:", em.out["cpu_baseline"]["sample"])
 
     if args.compare and args.params == "learned" and elapsed() + 3 * em.out["value"] < budget:
-        # the same pencil with the best constant window of the training runs (AED window 48 at every
-        # n, other parameters PIPARMQ's): what the network adds over a static choice
+        # the same pencil with the best constant parameters of the training grid (AED window 64,
+        # sweep-skip threshold 40 %, NS PIPARMQ's): what the network adds over a static choice
         prm_static = gqz.GqzParams()
-        prm_static.nw = 48
+        prm_static.nw = 64
+        prm_static.nibble = 40
         H.copy_(H0); T.copy_(T0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, _, _, info_s, st_s = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm_static)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
-        em.out["learned_vs_static"] = {"n": n, "static_nw": 48, "static_s": ts, "static_info": info_s,
-                                       "learned_s": em.out["value"], "gain": ts / em.out["value"],
-                                       "note": "same pencil and call, AED window fixed at 48 (the fastest "
-                                               "constant window of the training runs), other parameters "
+        em.out["learned_vs_static"] = {"n": n, "static_nw": 64, "static_nibble": 40, "static_s": ts,
+                                       "static_info": info_s, "learned_s": em.out["value"],
+                                       "gain": ts / em.out["value"],
+                                       "note": "same pencil and call, AED window 64 and sweep-skip threshold "
+                                               "40 % fixed (the fastest constants of the training grid), NS "
                                                "PIPARMQ's"}
-        log("static NW=48: %.2f s (learned gain %.3fx)" % (ts, ts / em.out["value"]))
+        log("static NW=64 NIBBLE=40: %.2f s (learned gain %.3fx)" % (ts, ts / em.out["value"]))
 
     abandon = False
     if args.compare and args.params == "learned":

@@ -776,7 +776,7 @@ int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
                 };
                 int a = (int)std::lround(ns_def * mult(o[0], -2.0f, 1.0f));
                 int b = (int)std::lround(nw_def * mult(o[1], -5.0f, 1.0f));
-                int c = (int)std::lround(nib_def * mult(o[2], -2.0f, 2.0f));
+                int c = (int)std::lround(nib_def * mult(o[2], -2.0f, 4.0f));
                 ns_it = std::max(2, a - a % 2);
                 nw_it = std::max(8, b);
                 nib_it = std::min(100, std::max(1, c));

@@ -8,12 +8,13 @@ of n, the last deflation fractions, the previous choices, ...) to log2 multiplie
 values.  The QZ arithmetic is unchanged: only the three counts move, every result still has to pass
 the backward-error check.
 
-Training data come from the QZ itself: for each training pencil a small grid of static multipliers
-is run and every top-level iteration is recorded (gqz_trace_get).  The fastest run fixes the target:
-its first AED window W*; every recorded state of the pencil gets the NW target log2(W* / NW_def(n))
-(the defaults are evaluated once per QZ call, at the order n of the active block) and NS, NIBBLE
-targets 0.  Training uses the library's own FP32 gradient and Adam kernels
-(gqz_mlp_train_step_dev) on the GPU.
+Training data come from the QZ itself: for each training pencil a grid of phase policies (an AED
+window, and a sweep-skip threshold for the first half of the deflation and another for the second,
+PhasePolicy) is run and every top-level iteration is recorded (gqz_trace_get).  The fastest run that
+passes the backward-error check fixes the targets: every recorded state of the pencil gets the NW
+target log2(W* / NW_def(n)) and the NIBBLE target log2(NIBBLE*(phase of that state) / NIBBLE_def(n))
+(the defaults are evaluated once per QZ call, at the order n of the active block) and the NS target 0.
+Training uses the library's own FP32 gradient and Adam kernels (gqz_mlp_train_step_dev) on the GPU.
 """
 import ctypes as C
 import math
@@ -45,6 +46,13 @@ def piparmq_ns(nh):
 def piparmq_nw(nh):
     ns = piparmq_ns(nh)
     return ns if nh <= 500 else 3 * ns // 2
+
+
+def piparmq_nibble(nh):
+    """NIBBLE of PDHGEQZ0 (PIPARMQ ISPEC 52; 14 below NMIN1 = 6000)."""
+    if nh < 6000:
+        return 14
+    return min(100, int(math.ceil(335.0 * nh ** -0.44)))
 
 
 # ---- parameters --------------------------------------------------------------------------------------
@@ -130,6 +138,27 @@ def make_pencil(kind, n, seed):
 
 
 # ---- traces ---------------------------------------------------------------------------------------------
+class PhasePolicy:
+    """Host callback with an absolute AED window and a sweep-skip threshold that depends on the phase
+    of the iteration: nib_early while the active block holds more than half of the pencil (feature 1,
+    nh / n), nib_late afterwards.  NS keeps its default (the AED window bounds the shifts anyway)."""
+
+    def __init__(self, L, nw, nib_early, nib_late):
+        self.L = L
+
+        def cb(user, feat, nh, ns, nw_, nib):
+            nw_[0] = nw
+            nib[0] = nib_early if feat[1] > 0.5 else nib_late
+        self.fn = gqz.POLICY_FN(cb)
+
+    def __enter__(self):
+        self.L.gqz_set_policy(self.fn, None)
+        return self
+
+    def __exit__(self, *a):
+        self.L.gqz_set_policy(gqz.POLICY_FN(0), None)
+
+
 class StaticPolicy:
     """Host callback that multiplies the default NS, NW, NIBBLE (passed in by the library) by fixed
     factors (used to explore the parameter space while recording traces)."""
@@ -152,7 +181,7 @@ class StaticPolicy:
         self.L.gqz_set_policy(gqz.POLICY_FN(0), None)
 
 
-def run_traced(L, H0, T0, mult=None, learned=False, cap=100000):
+def run_traced(L, H0, T0, mult=None, learned=False, cap=100000, phase=None):
     """One QZ (JOB='S', COMPQ=COMPZ='I') of a copy of (H0, T0); returns (seconds, info, records,
     residuals), the residuals being PCOMPRES's [resA, resB, resQ/(n eps), resZ/(n eps)] of the result
     (computed after the timed call)."""
@@ -162,11 +191,14 @@ def run_traced(L, H0, T0, mult=None, learned=False, cap=100000):
     H.copy_(H0); T.copy_(T0)
     L.gqz_trace_enable(cap)
     prm = gqz.GqzParams()
-    if mult is not None or learned:
+    if mult is not None or learned or phase is not None:
         prm.use_policy = 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    if mult is not None:
+    if phase is not None:
+        with PhasePolicy(L, *phase):
+            _, _, _, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
+    elif mult is not None:
         with StaticPolicy(L, *mult):
             _, _, _, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
     else:

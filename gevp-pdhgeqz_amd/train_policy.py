@@ -1,15 +1,18 @@
 """Generate QZ traces on the GPU and train the learned parameter policy (writes mlp_policy.npz).
 
     python gevp-pdhgeqz_amd/train_policy.py [--sizes 2000,4000,8000] [--families ht,graded,singular]
-                                            [--seeds 1] [--steps 3000] [--out PATH]
+                                            [--seeds 1] [--large-sizes 12000,16000] [--steps 3000]
+                                            [--out PATH]
 
-For every training pencil the QZ runs once per multiplier triple of the grid (NS x NW x NIBBLE
-relative to the PIPARMQ defaults, which the QZ evaluates once per call at the order of the active
-block).  The first AED window W* of the fastest run is the target: every state recorded in the runs
-of that pencil is labelled log2(W* / NW_def(n)) for NW and 0 for NS and NIBBLE.  Larger pencils (--large-sizes, one family, a reduced grid) extend the
-range of active orders seen in training.  The perceptron is then fitted with
-the library's own FP32 gradient and Adam kernels.  Each run's backward error and orthogonality are
-checked (PCOMPRES on the device): a run that fails them, or returns INFO /= 0, is never a target.
+For every training pencil the QZ runs once per phase policy of the grid: an AED window NW and a
+sweep-skip threshold NIBBLE for the iterations while the active block holds more than half of the
+pencil, another for the rest (gqz_policy.PhasePolicy).  The fastest run that passes the checks fixes
+the targets of every state recorded in the runs of that pencil: log2(NW* / NW_def(n)) for NW,
+log2(NIBBLE*(phase of the state) / NIBBLE_def(n)) for NIBBLE, 0 for NS (the AED window bounds the
+shifts of a sweep, so NS above it has no effect).  Larger pencils (--large-sizes, one family) run a
+reduced grid.  The perceptron is then fitted with the library's own FP32 gradient and Adam kernels.
+Each run's backward error and orthogonality are checked (PCOMPRES on the device): a run that fails
+them, or returns INFO /= 0, is never a target.
 """
 import argparse
 import itertools
@@ -27,12 +30,12 @@ if HERE not in sys.path:
 import gqz  # noqa: E402
 import gqz_policy as GP  # noqa: E402
 
-# NS beyond the shifts an AED window delivers has no effect, so the grid spans the window size: from
-# windows that fit the LDS-resident AED kernel to windows with a nested QZ
-GRID_NS = (1.0,)
-GRID_NW = (0.03125, 0.0625, 0.125, 0.25)
-GRID_NIB = (1.0,)
-GRID_NW_LARGE = (0.0625, 0.125)
+# absolute AED windows (the LDS-resident AED kernel takes windows up to 64) and sweep-skip thresholds
+# in percent for the two phases of the deflation
+GRID_NW = (48, 64)
+GRID_NIB_EARLY = (20, 40)
+GRID_NIB_LATE = (20, 40, 60)
+GRID_NW_LARGE = (64,)
 
 
 def log(*a):
@@ -56,32 +59,41 @@ def main():
     fams = args.families.split(",")
     seeds = [int(x) for x in args.seeds.split(",")]
     X, Y, summary = [], [], []
-    cases = [(fam, n, seed, GRID_NW) for fam, n, seed in itertools.product(fams, sizes, seeds)]
+    # windows of 48 lose to 64 from n = 8000 on (measured): the larger orders run the 64 row only
+    cases = [(fam, n, seed, GRID_NW if n < 8000 else GRID_NW_LARGE)
+             for fam, n, seed in itertools.product(fams, sizes, seeds)]
     large = [int(x) for x in args.large_sizes.split(",") if x]
     cases += [(fam, n, seeds[0], GRID_NW_LARGE) for fam in args.large_families.split(",") for n in large]
     for fam, n, seed, grid_nw in cases:
         H0, T0 = GP.make_pencil(fam, n, 1000 * seed + n)
         runs = []
-        for mult in itertools.product(GRID_NS, grid_nw, GRID_NIB):
-            dt, info, rows, res = GP.run_traced(L, H0, T0, mult=mult)
-            runs.append((dt, info, mult, rows, res))
-            log("trace %s n=%d seed=%d mult=%s: %.2f s info=%d iterations=%d resA=%.1e resQ=%.2f"
-                % (fam, n, seed, mult, dt, info, len(rows), res[0], res[2]))
+        for phase in itertools.product(grid_nw, GRID_NIB_EARLY, GRID_NIB_LATE):
+            dt, info, rows, res = GP.run_traced(L, H0, T0, phase=phase)
+            runs.append((dt, info, phase, rows, res))
+            log("trace %s n=%d seed=%d NW=%d NIBBLE=%d/%d: %.2f s info=%d iterations=%d resA=%.1e resQ=%.2f"
+                % (fam, n, seed, phase[0], phase[1], phase[2], dt, info, len(rows), res[0], res[2]))
+        dflt_t, dflt_info, dflt_res = float("nan"), -1, None
+        if n <= 4000:                    # the PIPARMQ defaults for reference (minutes at larger orders)
+            dflt_t, dflt_info, _, dflt_res = GP.run_traced(L, H0, T0)
+            log("default %s n=%d: %.2f s info=%d" % (fam, n, dflt_t, dflt_info))
         ok = [r for r in runs if r[1] == 0 and GP.passes(r[4])]
         if not ok:
             log("no run of %s n=%d seed=%d passed the checks: pencil skipped" % (fam, n, seed))
             continue
         best = min(ok, key=lambda r: r[0])
-        dflt = [r for r in ok if r[2] == (1.0, 1.0, 1.0)]
-        wstar = best[3][0][3] if best[3] else GP.piparmq_nw(n)     # first AED window of the fastest run
+        nw_s, nib_e, nib_l = best[2]
+        t_nw = float(np.clip(np.log2(nw_s / GP.piparmq_nw(n)), -5.0, 1.0))
         for r in ok:                     # every recorded state of every run of this pencil
             for row in r[3]:
-                t_nw = float(np.clip(np.log2(wstar / GP.piparmq_nw(n)), -5.0, 1.0))
-                X.append(row[0]); Y.append(np.array([0.0, t_nw, 0.0], dtype=np.float32))
+                feat = row[0]
+                nib = nib_e if feat[1] > 0.5 else nib_l
+                t_nib = float(np.clip(np.log2(nib / GP.piparmq_nibble(n)), -2.0, 4.0))
+                X.append(feat); Y.append(np.array([0.0, t_nw, t_nib], dtype=np.float32))
         summary.append(dict(family=fam, n=n, seed=seed, best=list(best[2]), best_s=best[0],
-                            default_s=dflt[0][0] if dflt else None))
-        log("best %s n=%d: mult=%s %.2f s (default %.2f s)" % (fam, n, best[2], best[0],
-                                                               dflt[0][0] if dflt else float("nan")))
+                            default_s=dflt_t if dflt_info == 0 and GP.passes(dflt_res) else None,
+                            runs={"%d/%d/%d" % r[2]: r[0] for r in ok}))
+        log("best %s n=%d: NW=%d NIBBLE=%d/%d %.2f s (PIPARMQ defaults %.2f s)"
+            % (fam, n, nw_s, nib_e, nib_l, best[0], dflt_t))
         del H0, T0
         torch.cuda.empty_cache()
     X = np.array(X, dtype=np.float32); Y = np.array(Y, dtype=np.float32)
@@ -89,7 +101,9 @@ def main():
     t0 = time.time()
     P, losses = GP.train(L, X, Y, steps=args.steps, log=log)
     log("trained in %.1f s, final loss %.5f" % (time.time() - t0, losses[-1]))
-    GP.save(P, args.out, X=X, Y=Y, losses=np.array(losses), summary=json.dumps(summary))
+    # the committed model carries the parameters, the loss curve and the per-pencil summary (the
+    # training rows stay with the run that produced them)
+    GP.save(P, args.out, losses=np.array(losses), summary=json.dumps(summary), rows=np.array(X.shape[0]))
     log("wrote", args.out)
 
 

@@ -188,7 +188,7 @@ int gqz_small_qz_dev(int m, double *H, double *T, int ld, double *U, double *V,
  * A 12 -> 64 -> 64 -> 3 perceptron, GELU, FP32, parameters as one block of gqz_mlp_nparams_c()
  * floats: W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] (row-major).  Inputs: the 12
  * deflation-history features of a top-level iteration (gqz_trace_rec.feat); outputs: log2
- * multipliers of the default NS, NW and NIBBLE (clamped to [-2, 1], [-2, 1], [-2, 2]).
+ * multipliers of the default NS, NW and NIBBLE (clamped to [-2, 1], [-5, 1], [-2, 4]).
  * All pointers of the _dev calls are device pointers. */
 int gqz_mlp_nparams_c(void);
 int gqz_mlp_forward_dev(const float *params, const float *x, int batch, float *y);

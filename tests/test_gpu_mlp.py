@@ -80,8 +80,11 @@ def test_mlp_adam_training_fits_a_teacher():
 
 
 def test_learned_policy_versus_default():
-    """The trained policy on a held-out pencil: same backward-error bar as the default parameters,
-    and the QZ time with the learned NS / NW / NIBBLE is reported next to the default's."""
+    """The trained policy on a held-out pencil: same backward-error bar as the default parameters and
+    as the best constant parameters of the training grid (AED window 64, sweep-skip threshold 40 %);
+    the QZ time with the learned NS / NW / NIBBLE is reported next to both.  The policy must beat the
+    PIPARMQ defaults and stay within 15 % of the constant (it does not beat it: the training grid's
+    optimum is nearly the same at every order and phase)."""
     torch, gqz, O, L = _mods()
     import gqz_policy as GP
     if not os.path.exists(GP.MODEL):
@@ -90,7 +93,7 @@ def test_learned_policy_versus_default():
     print("learned vs default", n, flush=True)
     H0, T0 = GP.make_pencil("ht", n, 424242)
     res = {}
-    for mode in ("default", "learned", "default"):
+    for mode in ("default", "learned", "static", "default", "learned", "static"):
         if mode == "learned":
             GP.install_learned(L)
         else:
@@ -98,6 +101,9 @@ def test_learned_policy_versus_default():
         H = gqz.colmajor(n); T = gqz.colmajor(n); Q = gqz.colmajor(n); Z = gqz.colmajor(n)
         H.copy_(H0); T.copy_(T0)
         prm = GP.learned_params() if mode == "learned" else gqz.GqzParams()
+        if mode == "static":
+            prm.nw = 64
+            prm.nibble = 40
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, _, _, info, st = gqz.pdhgeqz("S", "I", "I", H, T, Q, Z, params=prm)
@@ -108,6 +114,7 @@ def test_learned_policy_versus_default():
         assert r[0] < 1e-13 and r[1] < 1e-13 and r[2] < 10 and r[3] < 10
         res[mode] = min(dt, res.get(mode, 1e30))
     GP.disable(L)
-    print("held-out n=%d: default %.2f s, learned %.2f s, gain %.2fx"
-          % (n, res["default"], res["learned"], res["default"] / res["learned"]), flush=True)
-    assert res["learned"] <= 1.05 * res["default"]
+    print("held-out n=%d: default %.2f s, learned %.2f s (gain %.2fx), constant NW=64 NIBBLE=40 %.2f s"
+          % (n, res["default"], res["learned"], res["default"] / res["learned"], res["static"]), flush=True)
+    assert res["learned"] < res["default"]
+    assert res["learned"] <= 1.15 * res["static"]
