@@ -2,9 +2,15 @@
 #pragma once
 #include "gqz_kernels.h"
 
+// kt0 >= 0: active block starts at kt0 (no search); chase = 0: an interior zero is reported in out[4]
+// instead of being chased (see k_misc.hip)
 void gqz_launch_inf_defl(hipStream_t st, double *H, double *T, double *Q, double *Z, int ld, int ldq,
                          int n, int nq, int ilo, int kbot, double ttol, double smlnum, double *ar,
-                         double *ai, double *be, int *out);
+                         double *ai, double *be, int *out, int kt0 = -1, int chase = 1);
+// the interior-zero chase through the LDS window [p, p + w), w <= gqz_zchase_max_window(): the zero at
+// T(p+1, p+1) ends at T(p+w-1, p+w-1); U, V (w x w, ld w) for the off-window updates
+int gqz_zchase_max_window();
+void gqz_launch_zchase_window(hipStream_t st, double *H, double *T, int ld, int p, int w, double *U, double *V);
 void gqz_launch_window_in(hipStream_t st, const double *H, const double *T, int ld, int k0, int jw,
                           double *Hw, double *Tw, double *U, double *V);
 void gqz_launch_window_out(hipStream_t st, double *H, double *T, int ld, int k0, int jw, const double *Hw,

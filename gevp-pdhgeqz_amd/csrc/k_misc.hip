@@ -8,13 +8,17 @@
 // max(TTOL, SMLNUM), PDHGEQZ7.f:380) are deflated: bottom zero by a column rotation, top zero
 // by a row rotation (beta = 0 exactly), interior zeros chased to the bottom first (DHGEQZ7 /
 // PDHGEQZ8 direction).  Rotations are applied to the full pencil (Schur form) and to Q, Z.
-//   out[0] = ktop, out[1] = kt, out[2] = kb (remaining active block), out[3] = #deflated
+// kt0 >= 0: the search for ktop is skipped and the active block starts at kt0 (a repeated call after
+// an interior zero was chased by the caller).  chase = 0: an interior zero is not chased here; the
+// kernel stops and reports its row in out[4] (the caller chases it in LDS windows, k_zchase_window).
+//   out[0] = ktop, out[1] = kt, out[2] = kb (remaining active block), out[3] = #deflated,
+//   out[4] = row of the interior zero left to the caller (-1: none)
 // ----------------------------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(1024) k_inf_defl(double *H, double *T, double *Q,
                                                               double *Z, int ld, int ldq, int n,
                                                               int nq, int ilo, int kbot, double ttol,
                                                               double smlnum, double *ar, double *ai,
-                                                              double *be, int *out) {
+                                                              double *be, int *out, int kt0, int chase) {
     __shared__ int s_i[8];
     __shared__ int s_max;
     __shared__ double s_d[4];
@@ -25,15 +29,17 @@ extern "C" __global__ void __launch_bounds__(1024) k_inf_defl(double *H, double 
     // found by all lanes at once
     if (tid == 0) s_max = ilo;
     __syncthreads();
-    for (int k = ilo + 1 + tid; k <= kbot; k += nt) {
-        double tst = fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]);
-        if (fabs(H[GQZ_IX(k, k - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) atomicMax(&s_max, k);
+    if (kt0 < 0) {
+        for (int k = ilo + 1 + tid; k <= kbot; k += nt) {
+            double tst = fabs(H[GQZ_IX(k - 1, k - 1, ld)]) + fabs(H[GQZ_IX(k, k, ld)]);
+            if (fabs(H[GQZ_IX(k, k - 1, ld)]) <= fmax(smlnum, GQZ_ULP * tst)) atomicMax(&s_max, k);
+        }
     }
     __syncthreads();
     if (tid == 0) {
-        const int ktop = s_max;
-        if (ktop > ilo) H[GQZ_IX(ktop, ktop - 1, ld)] = 0.0;
-        s_i[0] = ktop; s_i[1] = ktop; s_i[2] = kbot; s_i[3] = 0;
+        const int ktop = kt0 >= 0 ? kt0 : s_max;
+        if (kt0 < 0 && ktop > ilo) H[GQZ_IX(ktop, ktop - 1, ld)] = 0.0;
+        s_i[0] = ktop; s_i[1] = ktop; s_i[2] = kbot; s_i[3] = 0; s_i[6] = -1;
     }
     __syncthreads();
     const double tol = fmax(ttol, smlnum);
@@ -69,8 +75,10 @@ extern "C" __global__ void __launch_bounds__(1024) k_inf_defl(double *H, double 
                         ar[kt] = r; ai[kt] = 0.0; be[kt] = 0.0;
                         s_d[0] = c; s_d[1] = s; s_i[5] = kt;
                         s_i[1] = kt + 1; s_i[3]++; s_i[4] = 2;
-                    } else {
+                    } else if (chase) {
                         s_i[5] = found; s_i[4] = 3;
+                    } else {
+                        s_i[6] = found;     // left to the caller; op stays 0 (done)
                     }
                 }
             }
@@ -111,7 +119,91 @@ extern "C" __global__ void __launch_bounds__(1024) k_inf_defl(double *H, double 
         }
         __syncthreads();
     }
-    if (tid == 0) { out[0] = s_i[0]; out[1] = s_i[1]; out[2] = s_i[2]; out[3] = s_i[3]; }
+    if (tid == 0) { out[0] = s_i[0]; out[1] = s_i[1]; out[2] = s_i[2]; out[3] = s_i[3]; out[4] = s_i[6]; }
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_zchase_window: chase a zero diagonal entry of T down through the LDS-resident window
+// [p, p + w) of (H, T), w <= 64 (the interior-zero chase of k_inf_defl, DHGEQZ7 / PDHGEQZ8 on an NB
+// window, applied by the caller through PDHGEQZ6's updates).  T(p + 1, p + 1) = 0 on entry; each
+// step j = 1 .. w - 2 (window rows) moves it from j to j + 1 by the row rotation of rows (j, j + 1)
+// that zeroes T(j + 1, j + 1), then removes the fill H(j + 1, j - 1) with the column rotation of
+// columns (j, j - 1); on exit the zero sits at T(p + w - 1, p + w - 1).  The rotations are those of
+// k_inf_defl's chase in the same order; lane 0 forms both and applies them to the 2x2 corner of H
+// they share, the workgroup applies them to the rest of the window and accumulates U (row
+// rotations, as columns) and V (column rotations); the caller applies U to the rows of the window
+// right of it and to Q, V to the columns above it and to Z.  U, V: w x w, ld w.
+// ----------------------------------------------------------------------------------------------
+#define ZC_MAXW 64
+#define ZC_NT 256
+extern "C" __global__ void __launch_bounds__(ZC_NT) k_zchase_window(double *H, double *T, int ld, int p, int w,
+                                                                     double *U, double *V) {
+    extern __shared__ double lds[];
+    __shared__ double rot[4];
+    const int tid = threadIdx.x, nt = blockDim.x, lw = w + 1;
+    double *sH = lds, *sT = sH + (size_t)lw * w, *sU = sT + (size_t)lw * w, *sV = sU + (size_t)lw * w;
+    const double *gH = H + GQZ_IX(p, p, ld), *gT = T + GQZ_IX(p, p, ld);
+    for (int idx = tid; idx < w * w; idx += nt) {
+        const int i = idx % w, j = idx / w;
+        sH[i + lw * j] = gH[GQZ_IX(i, j, ld)];
+        sT[i + lw * j] = gT[GQZ_IX(i, j, ld)];
+        sU[i + lw * j] = (i == j) ? 1.0 : 0.0;
+        sV[i + lw * j] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int j = 1; j <= w - 2; ++j) {
+        if (tid == 0) {
+            double c, s, r, c2, s2, r2;
+            d_lartg(sT[j + lw * (j + 1)], sT[(j + 1) + lw * (j + 1)], c, s, r);
+            sT[j + lw * (j + 1)] = r; sT[(j + 1) + lw * (j + 1)] = 0.0;
+            // corner rows (j, j+1) x columns (j-1, j) of H: row rotation, then the column rotation
+            // formed from it (x' = c x + s y, y' = c y - s x on rows; on columns x = column j)
+            const double ha = sH[j + lw * (j - 1)], hb = sH[(j + 1) + lw * (j - 1)];
+            const double hc = sH[j + lw * j], hd = sH[(j + 1) + lw * j];
+            const double a1 = c * ha + s * hb, b1 = c * hb - s * ha;
+            const double c1 = c * hc + s * hd, d1 = c * hd - s * hc;
+            d_lartg(d1, b1, c2, s2, r2);
+            sH[(j + 1) + lw * j] = r2; sH[(j + 1) + lw * (j - 1)] = 0.0;
+            sH[j + lw * j] = c2 * c1 + s2 * a1; sH[j + lw * (j - 1)] = c2 * a1 - s2 * c1;
+            rot[0] = c; rot[1] = s; rot[2] = c2; rot[3] = s2;
+        }
+        __syncthreads();
+        const double c = rot[0], s = rot[1], c2 = rot[2], s2 = rot[3];
+        // rows (j, j+1): H columns j+1.., T columns j+2.., U columns (j, j+1); columns (j, j-1): H
+        // and T rows 0..j-1, V columns (j, j-1)
+        const int nlh = w - (j + 1), nlt = w - (j + 2), nr = j;
+        const int tot = nlh + nlt + w + 2 * nr + w;
+        for (int idx = tid; idx < tot; idx += nt) {
+            int t = idx;
+            if (t < nlh + nlt) {
+                double *M = t < nlh ? sH : sT;
+                const int col = t < nlh ? j + 1 + t : j + 2 + (t - nlh);
+                const double x = M[j + lw * col], y = M[(j + 1) + lw * col];
+                M[j + lw * col] = c * x + s * y; M[(j + 1) + lw * col] = c * y - s * x;
+                continue;
+            }
+            t -= nlh + nlt;
+            if (t < w) {
+                const double x = sU[t + lw * j], y = sU[t + lw * (j + 1)];
+                sU[t + lw * j] = c * x + s * y; sU[t + lw * (j + 1)] = c * y - s * x;
+                continue;
+            }
+            t -= w;
+            double *M = (t < nr) ? sH : (t < 2 * nr ? sT : sV);
+            const int row = (t < nr) ? t : (t < 2 * nr ? t - nr : t - 2 * nr);
+            const double x = M[row + lw * j], y = M[row + lw * (j - 1)];
+            M[row + lw * j] = c2 * x + s2 * y; M[row + lw * (j - 1)] = c2 * y - s2 * x;
+        }
+        __syncthreads();
+    }
+    double *oH = H + GQZ_IX(p, p, ld), *oT = T + GQZ_IX(p, p, ld);
+    for (int idx = tid; idx < w * w; idx += nt) {
+        const int i = idx % w, j = idx / w;
+        oH[GQZ_IX(i, j, ld)] = sH[i + lw * j];
+        oT[GQZ_IX(i, j, ld)] = sT[i + lw * j];
+        U[i + (size_t)w * j] = sU[i + lw * j];
+        V[i + (size_t)w * j] = sV[i + lw * j];
+    }
 }
 
 // copy the jw x jw window at (k0, k0) into contiguous Hw/Tw (zero below the Hessenberg /
@@ -203,9 +295,16 @@ static int nblk(size_t total) { size_t b = (total + 255) / 256; return (int)(b >
 
 void gqz_launch_inf_defl(hipStream_t st, double *H, double *T, double *Q, double *Z, int ld, int ldq,
                          int n, int nq, int ilo, int kbot, double ttol, double smlnum, double *ar,
-                         double *ai, double *be, int *out) {
+                         double *ai, double *be, int *out, int kt0, int chase) {
     hipLaunchKernelGGL(k_inf_defl, dim3(1), dim3(1024), 0, st, H, T, Q, Z, ld, ldq, n, nq, ilo, kbot,
-                       ttol, smlnum, ar, ai, be, out);
+                       ttol, smlnum, ar, ai, be, out, kt0, chase);
+}
+int gqz_zchase_max_window() { return ZC_MAXW; }
+void gqz_launch_zchase_window(hipStream_t st, double *H, double *T, int ld, int p, int w, double *U, double *V) {
+    const size_t lds = sizeof(double) * 4 * (size_t)(w + 1) * w;
+    static size_t lds_set[64] = {};
+    gqz_need_lds((const void *)k_zchase_window, lds, lds_set);
+    hipLaunchKernelGGL(k_zchase_window, dim3(1), dim3(ZC_NT), lds, st, H, T, ld, p, w, U, V);
 }
 void gqz_launch_window_in(hipStream_t st, const double *H, const double *T, int ld, int k0, int jw,
                           double *Hw, double *Tw, double *U, double *V) {

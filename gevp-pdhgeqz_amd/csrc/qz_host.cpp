@@ -719,6 +719,24 @@ void features(Run &R, int nh, float f[12]) {
     f[11] = (float)(std::log2((double)std::max(n, 2)) / 16.0);
 }
 
+// Interior zero T(z, z) of the active block [., kb] chased down to T(kb, kb) (the chase of
+// DHGEQZ7 / PDHGEQZ8 in NB windows, updates as PDHGEQZ6): each window [z - 1, z - 1 + w) holds the
+// zero in its second row and moves it to its last row, then U, V update the pencil outside it.
+void zero_chase(Run &R, const View &v, int z, int kb) {
+    const int WMAX = gqz_zchase_max_window();
+    size_t mk = L.ws.mark();
+    double *U = L.ws.alloc((size_t)WMAX * WMAX), *V = L.ws.alloc((size_t)WMAX * WMAX);
+    while (z < kb) {
+        const int p = z - 1, w = std::min(WMAX, kb - p + 1);
+        int ki = kt_begin(GQZ_KC_INF);
+        gqz_launch_zchase_window(L.st, v.H, v.T, v.ld, p, w, U, V);
+        kt_end(ki, 0.0, 64.0 * w * w);
+        apply_window(R, v, p, w, U, V);
+        z = p + w - 1;
+    }
+    L.ws.release(mk);
+}
+
 int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
     KtLevel klev(level);
     int nmin = R.prm.nmin > 0 ? std::min(R.prm.nmin, 64) : 64;
@@ -737,13 +755,23 @@ int qz_run(Run &R, const View &v, int ilo, int ihi, int level) {
         if (it >= maxit) return kbot + 1;
         double ti = now();
         R.st.iters++;
-        int ki = kt_begin(GQZ_KC_INF);
-        gqz_launch_inf_defl(L.st, v.H, v.T, v.Q, v.Z, v.ld, v.ldq, v.n, v.nq, ilo, kbot, R.ttol, v.smlnum, v.ar,
-                            v.ai, v.be, L.d_out);
-        kt_end(ki, 0.0, 32.0 * (kbot - ilo + 1));
-        sync_read(L.h_out, L.d_out, 4 * sizeof(int));
+        // deflation scan and infinite eigenvalues; an interior zero of diag(T) is chased to the bottom
+        // of the active block in LDS windows (zero_chase) and the scan resumes below the active top
+        int ktop = -1, kt = 0, kb = 0, ninf = 0, kt0 = -1, kb_cur = kbot;
+        for (;;) {
+            int ki = kt_begin(GQZ_KC_INF);
+            gqz_launch_inf_defl(L.st, v.H, v.T, v.Q, v.Z, v.ld, v.ldq, v.n, v.nq, ilo, kb_cur, R.ttol, v.smlnum,
+                                v.ar, v.ai, v.be, L.d_out, kt0, 0);
+            kt_end(ki, 0.0, 32.0 * (kb_cur - ilo + 1));
+            sync_read(L.h_out, L.d_out, 5 * sizeof(int));
+            if (ktop < 0) ktop = L.h_out[0];
+            kt = L.h_out[1]; kb = L.h_out[2]; ninf += L.h_out[3];
+            const int zr = L.h_out[4];
+            if (zr < 0) break;
+            zero_chase(R, v, zr, kb);
+            kt0 = kt; kb_cur = kb;
+        }
         R.st.t_inf += now() - ti;
-        int ktop = L.h_out[0], kt = L.h_out[1], kb = L.h_out[2], ninf = L.h_out[3];
         if (ninf > 0) {
             R.st.inf_defl += ninf;
             if (top) R.last_ndinf = ninf;
