@@ -474,6 +474,8 @@ def main():
         res_box = {}
 
         def run_default():
+            # the library context is per host thread: this thread's progress lines on stderr
+            L.gqz_progress(30.0)
             H.copy_(H0); T.copy_(T0)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -487,8 +489,12 @@ def main():
         t_d0 = time.perf_counter()
         th = threading.Thread(target=run_default, daemon=True)
         th.start()
+        t_note = time.perf_counter()
         while th.is_alive() and elapsed() < budget - 2.0:
             th.join(0.5)
+            if time.perf_counter() - t_note >= 30.0:
+                t_note = time.perf_counter()
+                log("default-parameter run: %.0f s so far" % (t_note - t_d0))
         if "t" in res_box:
             cmp_["default_s"] = res_box["t"]
             cmp_["default_info"] = res_box["info"]
