@@ -479,11 +479,15 @@ hipEvent_t sync_event(size_t i) {
     return L.sev[i];
 }
 
+// blocks below this order (the nested QZ of AED windows, the end of a top-level QZ) keep the updates on
+// the chase stream: their slab updates are short and the stream ordering would cost more than it hides
+const int kOverlapMin = 2048;
+
 void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> &ch, const double *d_sr,
                   const double *d_si, const double *d_sb) {
     const int WMAX = gqz_chase_max_window();
     const int nslot = GQZ_MAXCH;
-    const bool ovl = L.overlap && L.st2;
+    const bool ovl = L.overlap && L.st2 && kbot - ktop + 1 >= kOverlapMin;
     size_t mk = L.ws.mark();
     const size_t slot = (size_t)WMAX * WMAX;
     double *U = L.ws.alloc(2 * (size_t)nslot * slot), *V = L.ws.alloc(2 * (size_t)nslot * slot);
