@@ -106,6 +106,30 @@ def ht_random_device(n, seed, torch, gqz):
     return H, T
 
 
+def stress_device(n, seed, torch, gqz, zero_frac=0.05, nnz_row=10, grade=3.0):
+    """The stress pencil of the benchmark configurations, generated on the device: A = sparse(nnz_row
+    N(0,1) entries per row at uniform random columns; a rare collision keeps one value) + 0.5 I, B upper
+    triangular N(0,1) with zero_frac of its diagonal set to exactly 0 at seeded positions, rows and
+    columns of both graded by the same factors 10^U(-grade, grade).  Column-major."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A = torch.zeros((n, n), dtype=torch.float64, device="cuda")        # row-major view of A^T
+    rows = torch.arange(n, device="cuda").repeat_interleave(nnz_row)
+    cols = torch.randint(0, n, (n * nnz_row,), device="cuda", generator=g)
+    vals = torch.randn(n * nnz_row, dtype=torch.float64, device="cuda", generator=g)
+    A[cols, rows] = vals                                               # A(row, col) stored transposed
+    idx = torch.arange(n, device="cuda")
+    A[idx, idx] += 0.5
+    B = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g).tril_()   # B^T lower
+    zi = torch.randperm(n, device="cuda", generator=g)[:max(1, int(round(zero_frac * n)))]
+    B[zi, zi] = 0.0
+    dr = 10.0 ** (torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 * grade - grade)
+    dc = 10.0 ** (torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 * grade - grade)
+    # stored transposed: entry (i, j) of the pencil sits at [j, i] of the row-major tensors
+    A.mul_(dc[:, None]).mul_(dr[None, :]); B.mul_(dc[:, None]).mul_(dr[None, :])
+    return A.t(), B.t()                                                # column-major n x n views
+
+
 def batched_workload(count, seed, steps, warmup, tune, torch, gqz, L):
     """count independent N(0,1) pencils with n = round(32 * 32^u), u ~ U(0,1) (log-uniform in
     [32, 1024]): QR(B) + HT reduction + QZ with Q and Z, one persistent launch replayed as a hipGraph,
@@ -206,7 +230,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--params", choices=["default", "learned"], default="learned")
+    ap.add_argument("--params", choices=["default", "learned"], default=None,
+                    help="QZ parameters (default: learned for the qz workload, PIPARMQ's for the pipeline)")
     ap.add_argument("--budget-s", type=float, default=470.0,
                     help="wall-clock budget from process start; the JSON line is printed before it ends")
     ap.add_argument("--step-share", type=float, default=0.15,
@@ -222,6 +247,9 @@ def main():
     ap.add_argument("--ktime-every", type=int, default=16,
                     help="time one launch in k (pseudo-random) of each kernel class (events cost queue packets)")
     ap.add_argument("--workload", choices=["qz", "batched", "pipeline"], default="qz")
+    ap.add_argument("--pencil", choices=["randn", "stress"], default="randn",
+                    help="pipeline workload: dense N(0,1) pencil, or the stress pencil (sparse + 0.5 I, 5 %% "
+                         "zero diagonal entries in triangular B, graded)")
     ap.add_argument("--tune", type=int, default=1, help="run the batched autotuner first")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first fill, then sum, a device buffer of this many MiB (kernels of known byte "
@@ -232,6 +260,8 @@ def main():
               "design); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     budget = args.budget_s
+    if args.params is None:
+        args.params = "default" if args.workload == "pipeline" else "learned"
 
     import torch
     import gqz
@@ -263,10 +293,18 @@ def main():
         # the full pipeline of a dense pencil (blocked QR of B, blocked HT reduction, multishift QZ with
         # the PIPARMQ defaults), stage by stage: the n = 8000 "full pipeline" configuration
         n = args.n
-        g = torch.Generator(device="cuda")
-        g.manual_seed(args.seed)
-        A0 = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g).t().contiguous().t()
-        B0 = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g).t().contiguous().t()
+        if args.pencil == "stress":
+            A0, B0 = stress_device(n, args.seed, torch, gqz)
+        else:
+            g = torch.Generator(device="cuda")
+            g.manual_seed(args.seed)
+            A0 = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g).t().contiguous().t()
+            B0 = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g).t().contiguous().t()
+        prm_p = gqz.GqzParams()
+        if args.params == "learned":
+            import gqz_policy
+            gqz_policy.install_learned(L)
+            prm_p = gqz_policy.learned_params()
         steps = []
         for s_ in range(max(1, args.steps)):
             S, P = A0.clone(), B0.clone()
@@ -276,26 +314,34 @@ def main():
             gqz.geqrb("I", S, P, Q)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
+            log("pipeline step %d: QR of B %.2f s" % (s_, t1 - t0))
             gqz.pdgghrd("V", "I", S, P, Q, Z)
             torch.cuda.synchronize()
             t2 = time.perf_counter()
-            _, _, _, info, st = gqz.pdhgeqz("S", "V", "V", S, P, Q, Z)
+            log("pipeline step %d: HT reduction %.2f s" % (s_, t2 - t1))
+            _, _, be_p, info, st = gqz.pdhgeqz("S", "V", "V", S, P, Q, Z, params=prm_p)
             torch.cuda.synchronize()
             t3 = time.perf_counter()
             steps.append({"qr_s": t1 - t0, "ht_s": t2 - t1, "qz_s": t3 - t2, "total_s": t3 - t0, "info": info,
-                          "sweeps": st["sweeps"], "aed_calls": st["aed_calls"]})
+                          "sweeps": st["sweeps"], "aed_calls": st["aed_calls"], "inf_defl": st["inf_defl"],
+                          "n_infinite": int((be_p == 0).sum().item())})
             log("pipeline step %d: %s" % (s_, steps[-1]))
             if elapsed() + 2 * (t3 - t0) > args.budget_s:
                 break
         res = gqz.compres(A0, B0, S, P, Q, Z)
         best = min(steps, key=lambda d: d["total_s"])
-        out = {"metric": "full pipeline seconds (QR of B + HT reduction + multishift QZ, PIPARMQ defaults)",
+        out = {"metric": "full pipeline seconds (QR of B + HT reduction + multishift QZ, %s parameters)"
+                         % ("PIPARMQ default" if args.params == "default" else "learned"),
                "value": best["total_s"], "unit": "s", "n_gpus": 1, "steps": len(steps), "warmup": 0,
                "ms_per_step": 1000.0 * sum(d["total_s"] for d in steps) / len(steps), "higher_is_better": False,
                "scaling": "strong", "vs_baseline": None, "dtype": "f64",
-               "data": "synthetic: A, B i.i.d. N(0,1) generated on the device, seed %d" % args.seed,
-               "config": {"workload": "dense pencil, n=%d, QR(B) + PDGGHRD + PDHGEQZ (JOB=S, COMPQ=COMPZ=V)" % n,
-                          "n": n, "stages": best},
+               "data": ("synthetic: A, B i.i.d. N(0,1) generated on the device, seed %d" % args.seed
+                        if args.pencil == "randn" else
+                        "synthetic stress pencil generated on the device (sparse A + 0.5 I, 5 %% zero diagonal "
+                        "entries in triangular B, rows and columns graded by 10^U(-3,3)), seed %d" % args.seed),
+               "config": {"workload": "%s pencil, n=%d, QR(B) + PDGGHRD + PDHGEQZ (JOB=S, COMPQ=COMPZ=V)"
+                                      % ("dense" if args.pencil == "randn" else "stress", n),
+                          "n": n, "pencil": args.pencil, "params": args.params, "stages": best},
                "check": {"resA": res[0], "resB": res[1], "resQ_neps": res[2], "resZ_neps": res[3]},
                "roofline": None, "cpu_baseline": None}
         print(json.dumps(out), flush=True)

@@ -26,8 +26,10 @@ void gqz_launch_gghrd_wg(hipStream_t st, double *A, double *B, int ld, int n, in
 // Givens HT reduction with the rotations of each column sweep applied in bulk (k_ht_blk.hip); ws holds
 // gqz_gghrd_blk_ws_bytes(n, nq) bytes (rotation list, progress counters, per-block carries)
 size_t gqz_gghrd_blk_ws_bytes(int n, int nq);
+// (the column sweeps j = ilo .. min(jlast, ihi - 2); jlast < 0: all of them; a call for a later range
+// continues the reduction, since a sweep depends only on j and ihi)
 void gqz_launch_gghrd_blk(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
-                          double *Z, int ldq, int nq, void *ws);
+                          double *Z, int ldq, int nq, void *ws, int jlast = -1);
 void gqz_launch_house_qr(hipStream_t st, int n, double *A, int lda, double *B, int ldb, double *Q, int ldq,
                          double *tau);
 // blocked QR of B (compact WY panels of 32 columns, GEMM updates); ws: gqz_house_qr_ws_doubles(n)

@@ -603,12 +603,13 @@ size_t gqz_gghrd_blk_ws_bytes(int n, int nq) {
 }
 
 void gqz_launch_gghrd_blk(hipStream_t st, double *A, double *B, int ld, int n, int ilo, int ihi, double *Q,
-                          double *Z, int ldq, int nq, void *ws) {
+                          double *Z, int ldq, int nq, void *ws, int jlast) {
     if (ihi - ilo < 2) return;
     const HtWs w = ht_carve(ws, n, nq > n ? nq : n, nullptr);
     GQZ_HIP(hipMemsetAsync(w.ctr, 0, sizeof(unsigned long long) * (size_t)(1 + ht_nhmax(n)), st));
     const int cap = ht_coop_capacity();
-    for (int j = ilo; j <= ihi - 2; ++j) {
+    const int jend = (jlast >= 0 && jlast < ihi - 2) ? jlast : ihi - 2;
+    for (int j = ilo; j <= jend; ++j) {
         HtSweep S;
         S.j = j; S.ihi = ihi; S.nb = (ihi - j - 1 + HT_K - 1) / HT_K;
         unsigned long long tag = (unsigned long long)(j - ilo + 1) << 32;

@@ -1203,8 +1203,24 @@ int gqz_gghrd_dev(char compq, char compz, int n, int ilo, int ihi, double *A, in
             if (ihi - ilo + 1 > kHtBlockedMin) {
                 DevBuf rws;
                 rws.alloc_async(gqz_gghrd_blk_ws_bytes(n, n), L.st);
-                gqz_launch_gghrd_blk(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
-                                     icq ? ldq : ldz, n, rws.p);
+                // with progress lines on, the column sweeps go in chunks with a line after each
+                const int chunk = L.progress_s > 0.0 ? 256 : n;
+                const double t0 = now();
+                double tp = t0;
+                for (int j0 = ilo - 1; j0 <= ihi - 3; j0 += chunk) {
+                    gqz_launch_gghrd_blk(L.st, A, B, lda, n, j0, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
+                                         icq ? ldq : ldz, n, rws.p, j0 + chunk - 1);
+                    if (L.progress_s > 0.0) {
+                        HIPCHK(hipStreamSynchronize(L.st));
+                        const double t = now();
+                        if (t - tp >= L.progress_s) {
+                            tp = t;
+                            std::fprintf(stderr, "[gqz] HT reduction n=%d: %d of %d column sweeps, %.1f s\n", n,
+                                         std::min(j0 + chunk, ihi - 2) - (ilo - 1), ihi - ilo - 1, t - t0);
+                            std::fflush(stderr);
+                        }
+                    }
+                }
                 HIPCHK(hipStreamSynchronize(L.st));
             } else {
                 gqz_launch_gghrd_wg(L.st, A, B, lda, n, ilo - 1, ihi - 1, icq ? Q : nullptr, icz ? Z : nullptr,
