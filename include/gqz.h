@@ -25,7 +25,7 @@ typedef struct {
     int ns;        /* shifts per sweep (ISPEC 53)                                               */
     int nw;        /* AED window (ISPEC 51)                                                     */
     int nibble;    /* sweep-skip threshold in percent (ISPEC 52)                                */
-    int chain;     /* shifts per bulge chain (SHIFTPERWND, PDHGEQZ0.f:1145-1148); <= 40         */
+    int chain;     /* shifts per bulge chain (SHIFTPERWND, PDHGEQZ0.f:1145-1148); <= 40; 0: NS/6 in [10, 20] */
     int maxit;     /* iteration cap factor (MAXIT = maxit * NH, 30 in the reference)           */
     int use_policy;/* 1: consult the installed parameter policy at every top-level iteration  */
 } gqz_params;
