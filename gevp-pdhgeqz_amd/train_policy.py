@@ -6,7 +6,8 @@
 
 For every training pencil the QZ runs once per phase policy of the grid: an AED window NW and a
 sweep-skip threshold NIBBLE for the iterations while the active block holds more than half of the
-pencil, another for the rest (gqz_policy.PhasePolicy).  The fastest run that passes the checks fixes
+pencil, another for the rest (gqz_policy.PhasePolicy).  The grid spans the region where a first,
+wider grid (NW 48/64, NIBBLE 20/40 early and 20/40/60 late, every order) found the fastest runs.  The fastest run that passes the checks fixes
 the targets of every state recorded in the runs of that pencil: log2(NW* / NW_def(n)) for NW,
 log2(NIBBLE*(phase of the state) / NIBBLE_def(n)) for NIBBLE, 0 for NS (the AED window bounds the
 shifts of a sweep, so NS above it has no effect).  Larger pencils (--large-sizes, one family) run a
@@ -34,7 +35,7 @@ import gqz_policy as GP  # noqa: E402
 # in percent for the two phases of the deflation
 GRID_NW = (48, 64)
 GRID_NIB_EARLY = (20, 40)
-GRID_NIB_LATE = (20, 40, 60)
+GRID_NIB_LATE = (40, 60)
 GRID_NW_LARGE = (64,)
 
 
@@ -49,7 +50,8 @@ def main():
     ap.add_argument("--seeds", default="1")
     ap.add_argument("--large-sizes", default="12000,16000")
     ap.add_argument("--large-families", default="ht")
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--rows-out", default=None, help="also write the training states and targets (npz)")
     ap.add_argument("--out", default=GP.MODEL)
     args = ap.parse_args()
     import torch
@@ -59,8 +61,8 @@ def main():
     fams = args.families.split(",")
     seeds = [int(x) for x in args.seeds.split(",")]
     X, Y, summary = [], [], []
-    # windows of 48 lose to 64 from n = 8000 on (measured): the larger orders run the 64 row only
-    cases = [(fam, n, seed, GRID_NW if n < 8000 else GRID_NW_LARGE)
+    # windows of 48 lose to 64 from n = 4000 on (measured): the larger orders run the 64 row only
+    cases = [(fam, n, seed, GRID_NW if n < 4000 else GRID_NW_LARGE)
              for fam, n, seed in itertools.product(fams, sizes, seeds)]
     large = [int(x) for x in args.large_sizes.split(",") if x]
     cases += [(fam, n, seeds[0], GRID_NW_LARGE) for fam in args.large_families.split(",") for n in large]
@@ -98,6 +100,8 @@ def main():
         torch.cuda.empty_cache()
     X = np.array(X, dtype=np.float32); Y = np.array(Y, dtype=np.float32)
     log("dataset: %d rows" % X.shape[0])
+    if args.rows_out:
+        np.savez(args.rows_out, X=X, Y=Y)
     t0 = time.time()
     P, losses = GP.train(L, X, Y, steps=args.steps, log=log)
     log("trained in %.1f s, final loss %.5f" % (time.time() - t0, losses[-1]))
