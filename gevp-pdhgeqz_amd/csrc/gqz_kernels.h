@@ -78,12 +78,16 @@ void gqz_chase_window(hipStream_t st, const GqzChainArgs &a);
 struct GqzChainSet {
     double *H, *T;
     int ld, nch;
-    double *U, *V;            // chain c: U + c * ustride (w_c x w_c, ld w_c); same for V
+    double *U, *V;            // chain c: U + uslot[c] * ustride (w_c x w_c, ld w_c); same for V
     int ustride, pad;
     const double *sr, *si, *sb;
     double smlnum;
     int p[GQZ_MAXCH], w[GQZ_MAXCH], nb[GQZ_MAXCH], t0[GQZ_MAXCH], steps[GQZ_MAXCH];
-    int intro[GQZ_MAXCH], last[GQZ_MAXCH], soff[GQZ_MAXCH];
+    int intro[GQZ_MAXCH], last[GQZ_MAXCH], soff[GQZ_MAXCH], uslot[GQZ_MAXCH];
+    // deferred left update of the chain's previous window (near[c] > 0): before its window is
+    // loaded, chain c applies U + pslot[c] * ustride (pw[c] x pw[c]) to rows [pp, pp + pw) of H and
+    // T in the near[c] columns right of that window, the columns its window reads
+    int near[GQZ_MAXCH], pslot[GQZ_MAXCH], pp[GQZ_MAXCH], pw[GQZ_MAXCH];
 };
 void gqz_chase_set(hipStream_t st, const GqzChainSet &s);
 int gqz_chase_max_window();

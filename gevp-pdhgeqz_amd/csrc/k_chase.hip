@@ -17,7 +17,10 @@
 // reference's sequential chase up to rounding.
 //
 // The parts of the pencil outside the window are updated afterwards by the MFMA slab kernels
-// with U (rows to the right) and V (columns above, Q, Z).
+// with U (rows to the right) and V (columns above, Q, Z).  When the host defers the left update of
+// the columns that a chain's next window reads (near[c] > 0), the chain's next launch applies it
+// while it loads its window (FP64 MFMA from the previous U staged in LDS), so that no separate
+// launch sits between two chases of a chain.
 #include "gqz_dev.h"
 #include "gqz_kernels.h"
 #include "gqz_misc.h"
@@ -25,6 +28,10 @@
 #define CH_MAXW 64
 #define CH_MAXB 21
 #define CH_NT 1024         // 16 waves: the LDS round trips of the update phases overlap
+#define CH_LDX 66          // column stride of the staged near block (2-way bank conflicts at most)
+#define CH_NEAR_LDS (sizeof(double) * (4096 + 2 * 64 * CH_LDX))
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     extern __shared__ double lds[];
@@ -32,7 +39,7 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     GqzChainArgs a;
     a.H = S.H; a.T = S.T; a.ld = S.ld; a.p = S.p[cidx]; a.w = S.w[cidx]; a.nb = S.nb[cidx];
     a.t0 = S.t0[cidx]; a.steps = S.steps[cidx]; a.intro = S.intro[cidx]; a.last = S.last[cidx];
-    a.U = S.U + (size_t)cidx * S.ustride; a.V = S.V + (size_t)cidx * S.ustride;
+    a.U = S.U + (size_t)S.uslot[cidx] * S.ustride; a.V = S.V + (size_t)S.uslot[cidx] * S.ustride;
     a.sr = S.sr + S.soff[cidx]; a.si = S.si + S.soff[cidx]; a.sb = S.sb + S.soff[cidx];
     a.smlnum = S.smlnum;
     const int w = a.w, ld = w + 1, nb = a.nb;
@@ -43,24 +50,97 @@ __global__ void __launch_bounds__(CH_NT) k_chase(GqzChainSet S) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *gH = a.H + GQZ_IX(a.p, a.p, a.ld);
     const double *gT = a.T + GQZ_IX(a.p, a.p, a.ld);
-    // window load: every load of a thread is issued before its first LDS write
-    {
-        double hv[CH_MAXW * CH_MAXW / CH_NT], tv[CH_MAXW * CH_MAXW / CH_NT];
+    // Deferred left update of the previous window [pp, pp + pw) (near columns, nc > 0): X <- U^T X for
+    // X = rows [pp, pp + pw) x columns [pp + pw, pp + pw + nc) of H and T, with this chain's previous U
+    // (pw x pw, ld pw).  The rows from p on (the first pw - d rows of this window, d = p - pp) are this
+    // window's block [0, pw - d) x [pw - d, w) and go to LDS; the rows above go back to HBM.  U, X and
+    // the window are loaded together (one memory latency), X by whole columns; U is staged in LDS (sF,
+    // 64 x 64) in the operand order of FP64 MFMA 16x16x4: entry (r, c) at
+    // ((c / 16) * 16 + r / 4) * 64 + (r % 4) * 16 + c % 16, zero beyond pw; X in LDS by columns (sX,
+    // leading dimension CH_LDX, H then T).  Work unit (matrix, 16-column tile ct, 16-row output block
+    // it), at most two per wave: lane (li, lk) = (l % 16, l / 16) takes X(4 s + lk, 16 ct + li),
+    // s = 0..15, and receives output rows 16 it + li of columns 16 ct + lk + 4 r.  Every wave has read
+    // its operands before the barrier after which the window overwrites sF and sX.
+    const int nc = S.near[cidx], pp = S.pp[cidx], pw = S.pw[cidx], d = a.p - pp, nb0 = pw - d;
+    const int nct = (nc + 15) >> 4, nunit = 8 * nct;      // nc <= 64: at most 32 units
+    double *XH = a.H + GQZ_IX(pp, pp + pw, a.ld), *XT = a.T + GQZ_IX(pp, pp + pw, a.ld);
+    const int li = lane & 15, lk = lane >> 4;
+    double uv[4096 / CH_NT], xv[2 * 64 * 64 / CH_NT];
+    dbl4 acc[2];
+    if (nc > 0) {
+        const double *Up = S.U + (size_t)S.pslot[cidx] * S.ustride;
 #pragma unroll
-        for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
-            const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
-            const bool ok = idx < w * w;
-            hv[q] = ok ? gH[GQZ_IX(i, j, a.ld)] : 0.0;
-            tv[q] = ok ? gT[GQZ_IX(i, j, a.ld)] : 0.0;
+        for (int q = 0; q < 4096 / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, r = idx & 63, c = idx >> 6;
+            uv[q] = (r < pw && c < pw) ? Up[r + pw * c] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
-            const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
-            if (idx < w * w) {
+        for (int q = 0; q < 2 * 64 * 64 / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, r = idx & 63, col = (idx >> 6) & 63, m = idx >> 12;
+            xv[q] = (r < pw && col < nc) ? (m == 0 ? XH : XT)[r + (long)col * a.ld] : 0.0;
+        }
+    }
+    // window: every load of a thread is issued before its first LDS write
+    double hv[CH_MAXW * CH_MAXW / CH_NT], tv[CH_MAXW * CH_MAXW / CH_NT];
+#pragma unroll
+    for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
+        const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
+        const bool ok = idx < w * w;
+        hv[q] = ok ? gH[GQZ_IX(i, j, a.ld)] : 0.0;
+        tv[q] = ok ? gT[GQZ_IX(i, j, a.ld)] : 0.0;
+    }
+    if (nc > 0) {
+        double *sF = lds, *sX = lds + 4096;
+#pragma unroll
+        for (int q = 0; q < 4096 / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, r = idx & 63, c = idx >> 6;
+            sF[(((c >> 4) * 16 + (r >> 2)) << 6) + (r & 3) * 16 + (c & 15)] = uv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * 64 * 64 / CH_NT; ++q) {
+            const int idx = tid + CH_NT * q, r = idx & 63, cm = idx >> 6;
+            sX[r + CH_LDX * cm] = xv[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int u = wave + (CH_NT / 64) * h;
+            acc[h] = dbl4{ 0.0, 0.0, 0.0, 0.0 };
+            if (u >= nunit) continue;
+            const int m = u / (4 * nct), rem = u - 4 * nct * m, ct = rem >> 2, it = rem & 3;
+            const double *xc = sX + CH_LDX * (64 * m + 16 * ct + li) + lk;
+#pragma unroll
+            for (int s2 = 0; s2 < 16; ++s2)
+                acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(xc[4 * s2], sF[((it * 16 + s2) << 6) + lane], acc[h], 0, 0, 0);
+        }
+        __syncthreads();               // sF and sX are read: the window may overwrite them
+    }
+#pragma unroll
+    for (int q = 0; q < CH_MAXW * CH_MAXW / CH_NT; ++q) {
+        const int idx = tid + CH_NT * q, i = idx % w, j = idx / w;
+        if (idx < w * w) {
+            if (!(nc > 0 && i < nb0 && j >= nb0)) {      // the near block comes from the update
                 sH[i + ld * j] = hv[q];
                 sT[i + ld * j] = tv[q];
-                sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
-                sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+            }
+            sU[i + ld * j] = (i == j) ? 1.0 : 0.0;
+            sV[i + ld * j] = (i == j) ? 1.0 : 0.0;
+        }
+    }
+    if (nc > 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int u = wave + (CH_NT / 64) * h;
+            if (u >= nunit) continue;
+            const int m = u / (4 * nct), rem = u - 4 * nct * m, ct = rem >> 2, it = rem & 3;
+            const int i = 16 * it + li;
+            if (i >= pw) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int cj = 16 * ct + lk + 4 * r;
+                if (cj >= nc) continue;
+                if (i >= d) (m == 0 ? sH : sT)[(i - d) + ld * (nb0 + cj)] = acc[h][r];
+                else (m == 0 ? XH : XT)[i + (long)cj * a.ld] = acc[h][r];
             }
         }
     }
@@ -265,8 +345,17 @@ static size_t chase_lds(int w) {
 void gqz_chase_set(hipStream_t st, const GqzChainSet &s) {
     if (s.nch <= 0) return;
     int wm = 0;
-    for (int c = 0; c < s.nch; ++c) wm = max(wm, s.w[c]);
+    bool nr = false;
+    for (int c = 0; c < s.nch; ++c) {
+        wm = max(wm, s.w[c]);
+        nr = nr || s.near[c] > 0;
+        // the near block must be exactly the block [0, pw - d) x [pw - d, w) of the window, d = p - pp
+        if (s.near[c] > 0 && (s.pw[c] > CH_MAXW || s.pw[c] < 1 || s.p[c] <= s.pp[c] || s.p[c] >= s.pp[c] + s.pw[c] ||
+                              s.pp[c] + s.pw[c] + s.near[c] != s.p[c] + s.w[c]))
+            throw std::runtime_error("chase: deferred update does not match the window");
+    }
     size_t lds = chase_lds(wm);
+    if (nr) lds = max(lds, CH_NEAR_LDS);      // the staged U and near block of the near update
     static size_t lds_set[64] = {};
     gqz_need_lds((const void *)k_chase, lds, lds_set);
     hipLaunchKernelGGL(k_chase, dim3(s.nch), dim3(CH_NT), lds, st, s);

@@ -84,6 +84,7 @@ struct Lib {
     // second stream (default priority) for the slab updates of a sweep, overlapped with the chase
     hipStream_t st2 = nullptr;
     bool overlap = true;
+    int overlap_min = 2048;     // smallest active block whose sweeps overlap
     std::vector<hipEvent_t> sev;     // ordering events between the two streams (no timing)
     Arena ws;
     int *d_out = nullptr, *h_out = nullptr;
@@ -455,20 +456,29 @@ int aed(Run &R, const View &v, int ktop, int kbot, int jw, int level, std::vecto
 // then windows starting where the previous one stopped), so the result does not depend on how
 // many chains share a round.
 //
-// Overlap (L.overlap): the updates of round r run on the second stream while round r + 1 is
-// chased.  Only the columns that a chain's next window reads ("near" columns [p + w, p' + w'),
-// p' = p + steps) get their left update on the chase stream before the next chase; the rest of
-// the left update ("far") and the right update (rows above, Q, Z) follow on the second stream:
-//     stream 1: chase(r) -> wait B(r-1) -> left-near(r) -> event A(r) -> chase(r+1) -> ...
+// Overlap (L.overlap): the updates of round r run on the second stream while later rounds are
+// chased.  The columns that a chain's next window reads ("near" columns [p + w, p' + w'),
+// p' = p + steps) get their left update from the chain's next launch itself (k_chase applies the
+// previous U to them before it loads its window); the rest of the left update ("far") and the
+// right update (rows above, Q, Z) go to the second stream:
+//     stream 1: chase(r) -> wait B(r-1) -> event A(r) -> chase(r+1) [near(previous) + windows] -> ...
 //     stream 2: wait A(r) -> left-far(r) -> right(r) -> event B(r)
-// chase(r+1) touches nothing that B(r) touches when no window of round r+1 reaches into the
-// columns [p, p + w) of a round-r window of a chain ahead of it (right(r) writes rows above p of
-// those columns): the scheduler keeps every window above the round-r window start of each chain
-// ahead.  left-near(r+1) waits for B(r): both update rows of the same window (far columns of
-// round r, the orders of U(r) and U(r+1) matter) and rows of the window behind.  U, V of
-// consecutive rounds use alternate slots, since B(r) still reads round r's while round r+1 is
-// chased.
-struct Chain { int soff, nb, gtop; bool started, done; int prevp; };
+// B(r) and chase(r+1) are released by the same event, so the chase is dispatched first (it follows the
+// event in its own queue) and gets its compute units before the update kernels fill the chip (its
+// workgroup needs a whole unit's LDS).  chase(r+1) runs concurrently with B(r), so it must touch
+// nothing that B(r) touches:
+//   * windows: the scheduler keeps every window of round r+1 above the round-r window start of each
+//     chain ahead (right(r) writes rows above p of the columns [p, p + w));
+//   * a chain's pending near block N (rows [p, p + w), columns [p + w, p' + w')): N ends above the
+//     window start of every chain ahead, now and later (checked when the chain is launched: chains
+//     only move down), so no right update ever reaches N while it is pending; a right update that
+//     did would mix columns with and without the chain's U;
+//   * chains behind a chain with a pending N stay above its row p until it moves again (its prevp
+//     persists), so their left updates, which do not commute with its U, never reach N's rows.
+// chase(r+1) waits for B(r-1): the near block of a chain overlaps the far block of its previous
+// window in rows.  U, V live in two slots per chain, alternating with the chain's launches: B(r)
+// reads round r's while the chain's next launch writes the other.
+struct Chain { int soff, nb, gtop; bool started, done; int prevp; int nl, pw, pnc; };
 
 hipEvent_t sync_event(size_t i) {
     while (L.sev.size() <= i) {
@@ -479,34 +489,68 @@ hipEvent_t sync_event(size_t i) {
     return L.sev[i];
 }
 
-// blocks below this order (the nested QZ of AED windows, the end of a top-level QZ) keep the updates on
-// the chase stream: their slab updates are short and the stream ordering would cost more than it hides
-const int kOverlapMin = 2048;
+// blocks below L.overlap_min (the nested QZ of AED windows, the end of a top-level QZ) keep the updates
+// on the chase stream: their slab updates are short and the stream ordering would cost more than it hides
 
 void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> &ch, const double *d_sr,
                   const double *d_si, const double *d_sb) {
     const int WMAX = gqz_chase_max_window();
     const int nslot = GQZ_MAXCH;
-    const bool ovl = L.overlap && L.st2 && kbot - ktop + 1 >= kOverlapMin;
+    const bool ovl = L.overlap && L.st2 && kbot - ktop + 1 >= L.overlap_min;
     size_t mk = L.ws.mark();
     const size_t slot = (size_t)WMAX * WMAX;
-    double *U = L.ws.alloc(2 * (size_t)nslot * slot), *V = L.ws.alloc(2 * (size_t)nslot * slot);
     const int nch = (int)ch.size();
-    for (auto &x : ch) x.prevp = INT32_MAX;
+    // without overlap: slots by launch position and round parity; with overlap: two per chain
+    const size_t nslots = ovl ? 2 * (size_t)nch : 2 * (size_t)nslot;
+    double *U = L.ws.alloc(nslots * slot), *V = L.ws.alloc(nslots * slot);
+    for (auto &x : ch) { x.prevp = INT32_MAX; x.nl = 0; x.pw = 0; x.pnc = 0; }
     int first = 0;                       // oldest chain not yet done
     long round = 0;
-    bool b_pending = false;              // stream 2 holds the updates of the previous round
+    bool b_pend[2] = { false, false };   // stream 2 holds updates recorded in B[parity]
+    auto wait_b = [&](int par) {
+        if (b_pend[par]) HIPCHK(hipStreamWaitEvent(L.st, sync_event(2 + par), 0));
+    };
+    // the previous round's windows, whose far-left and right updates are not launched yet
+    Win pws[GQZ_MAXCH];
+    int pnear[GQZ_MAXCH], pnw = 0, ppar = 0;
+    auto launch_b = [&]() {              // B(previous round), released by A on stream 1
+        if (pnw == 0) return;
+        HIPCHK(hipEventRecord(sync_event(0), L.st));
+        HIPCHK(hipStreamWaitEvent(L.st2, sync_event(0), 0));
+        GqzSlabJobs J;
+        J.njobs = 0;
+        for (int q = 0; q < pnw; ++q) {
+            const Win &x = pws[q];
+            const int c0 = x.p + x.w + pnear[q], ncol = v.n - c0;
+            if (J.njobs + 2 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, true, L.st2);
+            gqz_slab_add(J, v.H + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
+            gqz_slab_add(J, v.T + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
+        }
+        slab_launch(R, J, true, L.st2);
+        for (int q = 0; q < pnw; ++q) {
+            const Win &x = pws[q];
+            if (J.njobs + 4 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, false, L.st2);
+            gqz_slab_add(J, v.H + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+            gqz_slab_add(J, v.T + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
+            if (v.Q) gqz_slab_add(J, v.Q + (size_t)x.p * v.ldq, v.ldq, v.nq, x.U, x.w, x.w);
+            if (v.Z) gqz_slab_add(J, v.Z + (size_t)x.p * v.ldq, v.ldq, v.nq, x.V, x.w, x.w);
+        }
+        slab_launch(R, J, false, L.st2);
+        HIPCHK(hipEventRecord(sync_event(2 + ppar), L.st2));
+        b_pend[ppar] = true;
+        pnw = 0;
+    };
     while (first < nch) {
         const int par = (int)(round & 1);
-        double *Ur = U + (size_t)par * nslot * slot, *Vr = V + (size_t)par * nslot * slot;
         GqzChainSet S = {};
-        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = Ur; S.V = Vr; S.ustride = WMAX * WMAX;
+        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = U; S.V = V; S.ustride = WMAX * WMAX;
         S.sr = d_sr; S.si = d_si; S.sb = d_sb; S.smlnum = v.smlnum;
         Win ws[GQZ_MAXCH];
-        int slot_chain[GQZ_MAXCH];
+        int slot_chain[GQZ_MAXCH], nearc[GQZ_MAXCH];
         int nw = 0;
-        // rows from which the chains ahead forbid access
-        int limit = kbot + 1;
+        // rows from which the chains ahead forbid access; nlim: columns from which the windows of
+        // the chains ahead start, this round and later (a pending near block must end above it)
+        int limit = kbot + 1, nlim = kbot + 1;
         int active = 0;
         for (int c = 0; c < nch && nw < nslot; ++c) {
             Chain &x = ch[c];
@@ -518,96 +562,85 @@ void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> 
             if (!x.started && active >= nslot) break;
             int p = x.started ? x.gtop : ktop;
             int w = std::min(WMAX, kbot - p + 1);
-            if (p + w > limit) {
+            const int last = (p + w - 1 == kbot);
+            const int steps = last ? w : (w - 3 * x.nb - 1);
+            int nc = 0;
+            if (!last) {
+                const int np = p + steps, nw2 = std::min(WMAX, kbot - np + 1);
+                nc = std::max(0, std::min(np + nw2, v.n) - (p + w));
+            }
+            if (p + w > limit || (ovl && nc > 0 && p + w + nc > nlim)) {
                 // wait: the chain ahead still occupies rows below; nothing behind may pass
                 limit = std::min(limit, x.started ? x.gtop : ktop);
                 if (ovl) limit = std::min(limit, pp);
+                nlim = std::min(nlim, p);
                 if (!x.started) break;
                 ++active;
                 continue;
             }
-            const int last = (p + w - 1 == kbot);
             const int q = nw++;
             S.p[q] = p; S.w[q] = w; S.nb[q] = x.nb; S.soff[q] = x.soff;
             S.intro[q] = x.started ? 0 : 1; S.t0[q] = x.started ? 0 : -1; S.last[q] = last;
-            S.steps[q] = last ? w : (w - 3 * x.nb - 1);
-            ws[q] = Win{ p, w, Ur + (size_t)q * slot, Vr + (size_t)q * slot };
+            S.steps[q] = steps;
+            S.uslot[q] = ovl ? 2 * c + (x.nl & 1) : par * nslot + q;
+            if (ovl && x.pnc > 0) {
+                S.near[q] = x.pnc; S.pslot[q] = 2 * c + ((x.nl - 1) & 1); S.pp[q] = x.prevp; S.pw[q] = x.pw;
+            }
+            nearc[q] = ovl ? nc : 0;
+            ws[q] = Win{ p, w, U + (size_t)S.uslot[q] * slot, V + (size_t)S.uslot[q] * slot };
             slot_chain[q] = c;
             limit = p;
             if (ovl) limit = std::min(limit, pp);
+            nlim = std::min(nlim, p);
             ++active;
         }
         if (nw == 0) {
             bool pend = false;
-            for (auto &x : ch) pend = pend || x.prevp != INT32_MAX;
+            for (auto &x : ch) pend = pend || (x.done && x.prevp != INT32_MAX);
             if (!ovl || !pend) throw GqzError("chain scheduler made no progress");
-            // the only obstacle is the previous round's right update: let it finish first
-            HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
-            for (auto &x : ch) x.prevp = INT32_MAX;
+            // the only obstacle is the last right update of a finished chain: let stream 2 finish
+            launch_b();
+            wait_b(0); wait_b(1);
+            b_pend[0] = b_pend[1] = false;
+            for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
             continue;
         }
         S.nch = nw;
+        if (ovl) {
+            wait_b(par);                 // B(r-2)
+            launch_b();                  // B(r-1), released together with this chase
+        }
         double by = 0.0;
-        for (int q = 0; q < nw; ++q) by += 48.0 * S.w[q] * S.w[q];
+        for (int q = 0; q < nw; ++q) by += 48.0 * S.w[q] * S.w[q] + 32.0 * S.pw[q] * S.near[q];
         int kc = kt_begin(GQZ_KC_CHASE);
         gqz_chase_set(L.st, S);
         kt_end(kc, 0.0, by);
         R.st.chase_windows += nw;
-        for (auto &x : ch) x.prevp = INT32_MAX;
+        // a finished chain's window start constrains the next round only
+        for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
         if (!ovl) {
             apply_windows(R, v, ws, nw);
         } else {
-            // near columns first, on the chase stream, after the previous round's far/right updates
-            if (b_pending) HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
-            GqzSlabJobs J;
-            J.njobs = 0;
-            int nearc[GQZ_MAXCH];
-            for (int q = 0; q < nw; ++q) {
-                const Win &x = ws[q];
-                nearc[q] = 0;
-                if (!S.last[q]) {
-                    const int np = x.p + S.steps[q], nw2 = std::min(WMAX, kbot - np + 1);
-                    nearc[q] = std::max(0, std::min(np + nw2, v.n) - (x.p + x.w));
-                }
-                if (nearc[q] <= 0) continue;
-                gqz_slab_add(J, v.H + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, nearc[q], x.U, x.w, x.w);
-                gqz_slab_add(J, v.T + (size_t)x.p + (size_t)(x.p + x.w) * v.ld, v.ld, nearc[q], x.U, x.w, x.w);
-            }
-            slab_launch(R, J, true);
-            HIPCHK(hipEventRecord(sync_event(0), L.st));
-            HIPCHK(hipStreamWaitEvent(L.st2, sync_event(0), 0));
-            for (int q = 0; q < nw; ++q) {
-                const Win &x = ws[q];
-                const int c0 = x.p + x.w + nearc[q], ncol = v.n - c0;
-                if (J.njobs + 2 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, true, L.st2);
-                gqz_slab_add(J, v.H + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
-                gqz_slab_add(J, v.T + (size_t)x.p + (size_t)c0 * v.ld, v.ld, ncol, x.U, x.w, x.w);
-            }
-            slab_launch(R, J, true, L.st2);
-            for (int q = 0; q < nw; ++q) {
-                const Win &x = ws[q];
-                if (J.njobs + 4 > GQZ_SLAB_MAXJOBS) slab_launch(R, J, false, L.st2);
-                gqz_slab_add(J, v.H + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
-                gqz_slab_add(J, v.T + (size_t)x.p * v.ld, v.ld, x.p, x.V, x.w, x.w);
-                if (v.Q) gqz_slab_add(J, v.Q + (size_t)x.p * v.ldq, v.ldq, v.nq, x.U, x.w, x.w);
-                if (v.Z) gqz_slab_add(J, v.Z + (size_t)x.p * v.ldq, v.ldq, v.nq, x.V, x.w, x.w);
-            }
-            slab_launch(R, J, false, L.st2);
-            HIPCHK(hipEventRecord(sync_event(1), L.st2));
-            b_pending = true;
+            for (int q = 0; q < nw; ++q) { pws[q] = ws[q]; pnear[q] = nearc[q]; }
+            pnw = nw;
+            ppar = par;
         }
         for (int q = 0; q < nw; ++q) {
             Chain &x = ch[slot_chain[q]];
             x.started = true;
             x.gtop = S.p[q] + S.steps[q];
             x.prevp = S.p[q];
+            x.pw = S.w[q];
+            x.pnc = nearc[q];
+            ++x.nl;
             if (S.last[q]) x.done = true;
         }
         while (first < nch && ch[first].done) ++first;
         ++round;
     }
-    // everything after the sweep sees the updates of the last round
-    if (b_pending) HIPCHK(hipStreamWaitEvent(L.st, sync_event(1), 0));
+    // everything after the sweep sees the updates of every round
+    launch_b();
+    wait_b(0); wait_b(1);
     L.ws.release(mk);
 }
 
@@ -1095,6 +1128,11 @@ int gqz_progress(double seconds) { L.progress_s = seconds > 0.0 ? seconds : 0.0;
 int gqz_overlap(int on) {
     const int prev = L.overlap ? 1 : 0;
     if (on >= 0) L.overlap = on != 0;
+    return prev;
+}
+int gqz_overlap_min(int n) {
+    const int prev = L.overlap_min;
+    if (n >= 0) L.overlap_min = n;
     return prev;
 }
 int gqz_trace_get(gqz_trace_rec *out, int cap) {

@@ -67,6 +67,10 @@ int gqz_progress(double seconds);
  * products, in a different order).  on < 0 only queries.  Returns the previous setting; per host
  * thread, like the rest of the context. */
 int gqz_overlap(int on);
+/* Order of the smallest active block whose sweeps use the overlap (default 2048; smaller blocks, such as
+ * the nested QZs of AED windows, keep their updates on the library stream).  n < 0 only queries.
+ * Returns the previous value; per host thread.  No counterpart in the reference. */
+int gqz_overlap_min(int n);
 const char *gqz_last_error(void);
 
 /* ---- QZ ----------------------------------------------------------------------------------

@@ -349,6 +349,40 @@ def test_hgeqz_paths_with_aed_and_multishift():
     assert ratio < 1.0
 
 
+@pytest.mark.parametrize("chain", [0, 4, 20])
+def test_overlapped_sweep_matches_serial_order(chain):
+    """n=1500 with the overlapped sweep updates forced on for every active block (gqz_overlap_min):
+    the far left and the right updates of a round on the second stream while the next round is
+    chased, and each chain's near-column update applied by its next chase launch.  Chains of 10
+    shifts (and a 4-shift remainder), 4 and 20 shifts give different window steps and near widths.
+    The serial order on the same pencil is the comparison: both meet the residual bounds on the
+    host and match LAPACK's eigenvalues condition-weighted."""
+    torch, gqz, O = _mods()
+    L = gqz.lib()
+    n = 1500
+    _note("overlapped sweep, chain", chain, n)
+    H, T = QC.pencil_ht_random(n, 17)
+    prev_on, prev_min = L.gqz_overlap(-1), L.gqz_overlap_min(-1)
+    try:
+        L.gqz_overlap_min(64)
+        for ovl in (0, 1):
+            L.gqz_overlap(ovl)
+            Hd, Td = gqz.to_colmajor(torch.from_numpy(H)), gqz.to_colmajor(torch.from_numpy(T))
+            Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
+            prm = gqz.GqzParams()
+            prm.chain = chain
+            ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", Hd, Td, Qd, Zd, params=prm)
+            print("overlap", ovl, "info", info, "sweeps", st["sweeps"], "windows", st["chase_windows"], flush=True)
+            assert info == 0 and st["sweeps"] > 0
+            ra, rb, rq, rz = QC.compres(H, T, _host(Hd), _host(Td), _host(Qd), _host(Zd))
+            assert ra < RES_TOL and rb < RES_TOL and rq < ORTH_TOL and rz < ORTH_TOL, (ovl, ra, rb, rq, rz)
+            ratio, _, _ = QC.match_eigs(_host(ar), _host(ai), _host(be), H, T)
+            assert ratio < 1.0, (ovl, ratio)
+    finally:
+        L.gqz_overlap(prev_on)
+        L.gqz_overlap_min(prev_min)
+
+
 # ---- call-surface variants (JOB / COMPQ / COMPZ / ILO-IHI) ----------------------------------------
 def test_job_e_and_compq_n():
     torch, gqz, O = _mods()
