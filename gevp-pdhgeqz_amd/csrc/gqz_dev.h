@@ -79,12 +79,16 @@ __device__ __forceinline__ void d_larfg2(double &alpha, double &x1, double &tau)
 }
 
 // ---- short-latency variants for the one-lane chains of the chase and the window QZ ----------
-// A division compiles to a dozen dependent instructions and hypot to a library sequence; the
-// chains below use one hardware reciprocal refined by two Newton steps (the correctly rounded
-// reciprocal in all but rare cases, error below one ulp otherwise) and plain square roots of
-// sums of squares, with the exact routines outside the range where neither can under- or
-// overflow.  Every reflector keeps tau = 2 / (v^T v) of its own rounded v (one exact division),
-// so that it is orthogonal to working precision whatever the rounding of v.
+// A division compiles to a dozen dependent instructions, a square root to about as many (with
+// scaling for the extremes) and hypot to a library sequence; the chains below use one hardware
+// reciprocal refined by two Newton steps (the correctly rounded reciprocal in all but rare cases,
+// error below one ulp otherwise) and a square root from the hardware reciprocal square root with
+// one coupled Newton step and one residual correction (Markstein; error within one ulp, correctly
+// rounded in all but rare cases), on sums of squares in the range where neither can under- or
+// overflow (the exact routines elsewhere).  Every reflector takes tau = 2 / (v^T v) of its own
+// rounded v, so that it is orthogonal to working precision whatever the rounding of v; the
+// rounding of tau itself stays unbiased (round to nearest in all but rare cases), since a biased
+// tau makes long products of reflectors lose orthogonality coherently.
 
 // 1/x
 __device__ __forceinline__ double d_rcp(double x) {
@@ -96,16 +100,27 @@ __device__ __forceinline__ double d_rcp(double x) {
     return r;
 }
 
+// sqrt(x) for 1e-290 < x < 1e290 (no scaling needed): g ~ sqrt(x), h ~ 1 / (2 sqrt(x)) from the
+// hardware reciprocal square root; r = 1/2 - g h; g, h <- g + g r, h + h r; g <- g + (x - g^2) h
+__device__ __forceinline__ double d_sqrtf(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    return fma(fma(-g, g, x), h, g);
+}
+
 // DLARFG for [alpha; x1; x2] (the d_larfg3 reflector: v = [1; x / (alpha - beta)])
 __device__ __forceinline__ void d_larfg3f(double &alpha, double &x1, double &x2, double &tau) {
     const double m = fmax(fabs(alpha), fmax(fabs(x1), fabs(x2)));
     if (!(m > 1e-140 && m < 1e140)) { d_larfg3(alpha, x1, x2, tau); return; }
     const double s = fma(x1, x1, x2 * x2);
     if (s == 0.0) { tau = 0.0; return; }
-    const double beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+    const double beta = -copysign(d_sqrtf(fma(alpha, alpha, s)), alpha);
     const double rd = d_rcp(alpha - beta);
     x1 *= rd; x2 *= rd;
-    tau = 2.0 / fma(x1, x1, fma(x2, x2, 1.0));
+    tau = 2.0 * d_rcp(fma(x1, x1, fma(x2, x2, 1.0)));
     alpha = beta;
 }
 
@@ -115,7 +130,7 @@ __device__ __forceinline__ void d_larfg2f(double &alpha, double &x1, double &tau
     if (x1 == 0.0) { tau = 0.0; return; }
     const double m = fmax(fabs(alpha), fabs(x1));
     if (!(m > 1e-140 && m < 1e140)) { d_larfg2(alpha, x1, tau); return; }
-    const double beta = -copysign(sqrt(fma(alpha, alpha, x1 * x1)), alpha);
+    const double beta = -copysign(d_sqrtf(fma(alpha, alpha, x1 * x1)), alpha);
     x1 *= d_rcp(alpha - beta);
     tau = 2.0 * d_rcp(fma(x1, x1, 1.0));
     alpha = beta;
@@ -127,7 +142,7 @@ __device__ __forceinline__ void d_lartgf(double f, double g, double &c, double &
     if (f == 0.0) { c = 0.0; s = (g > 0.0) ? 1.0 : -1.0; r = fabs(g); return; }
     const double m = fmax(fabs(f), fabs(g));
     if (!(m > 1e-140 && m < 1e140)) { d_lartg(f, g, c, s, r); return; }
-    const double d = sqrt(fma(f, f, g * g));
+    const double d = d_sqrtf(fma(f, f, g * g));
     const double rd = d_rcp(d);
     c = fabs(f) * rd;
     s = (f > 0.0 ? g : -g) * rd;
@@ -202,7 +217,7 @@ __device__ __forceinline__ void d_invhse3f(double r10, double r11, double r12, d
     const double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
     const double rp = d_rcp(p0);
     const double p1 = w1 * rp, p2 = w2 * rp;
-    tau = 2.0 / fma(p1, p1, fma(p2, p2, 1.0));
+    tau = 2.0 * d_rcp(fma(p1, p1, fma(p2, p2, 1.0)));
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
@@ -222,10 +237,10 @@ __device__ __forceinline__ void d_nullrefl3(double r10, double r11, double r12, 
         d_invhse3f(r10, r11, r12, r20, r21, r22, v, tau);
         return;
     }
-    const double p0 = w0 + copysign(sqrt(nw), w0);
+    const double p0 = w0 + copysign(d_sqrtf(nw), w0);
     const double rp = d_rcp(p0);
     const double p1 = w1 * rp, p2 = w2 * rp;
-    tau = 2.0 / fma(p1, p1, fma(p2, p2, 1.0));
+    tau = 2.0 * d_rcp(fma(p1, p1, fma(p2, p2, 1.0)));
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 

@@ -69,13 +69,14 @@ __device__ __forceinline__ double rdlane(double x, int l) {
     return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 // Givens rotation with d_lartgf's arithmetic and conventions (c >= 0; r = sign(f) sqrt(f^2 + g^2),
-// f = 0 counting as positive) without branches: d = sqrt(f^2 + g^2) correctly rounded, 1/d from the
+// f = 0 counting as positive) without branches: d = sqrt(f^2 + g^2) from d_sqrtf, 1/d from the
 // hardware reciprocal and two Newton steps whose residual 1 - d r is exact (fma).  bad: operands whose
-// squares leave the safe range; the caller then takes d_lartgf's scaled path.
+// squares leave the safe range; the caller then takes d_lartgf's scaled path.  d, rd are used only
+// when f, g are both non-zero and not bad, where f^2 + g^2 lies in d_sqrtf's range.
 __device__ __forceinline__ void ht_lartg(double f, double g, double &c, double &s, double &r, bool &bad) {
     const double m = fmax(fabs(f), fabs(g));
     bad = (g != 0.0) && (f != 0.0) && !(m > 1e-140 && m < 1e140);
-    const double d = sqrt(fma(f, f, g * g));
+    const double d = d_sqrtf(fma(f, f, g * g));
     double rd = __builtin_amdgcn_rcp(d);
     rd = fma(rd, fma(-d, rd, 1.0), rd);
     rd = fma(rd, fma(-d, rd, 1.0), rd);
@@ -96,12 +97,17 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
-// progress counters: 64-bit, the sweep in the high half, so values only grow during a reduction
+// progress counters: 64-bit, the sweep in the high half, so values only grow during a reduction.
+// The polling loads are relaxed (an acquire load at device scope invalidates the L2 of the polling
+// XCD every time, which the other workgroups there, the lead among them, would pay for); one
+// acquire fence follows the load that meets the target.
 __device__ __forceinline__ void ht_wait(unsigned long long *p, unsigned long long target) {
-    while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
+// the release store makes the posting wave's earlier stores visible first; every value published
+// through a counter is written by the wave that posts it
 __device__ __forceinline__ void ht_post(unsigned long long *p, unsigned long long v) {
-    __threadfence();
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -138,11 +144,10 @@ __device__ void ht_helper(const double *B, int ld, const HtSweep S, const double
         }
         red[part][rr] = ps;
         __syncthreads();
-        if (part == 0 && use) {
+        if (part == 0 && use) {                   // wave 0, which also posts
             acc = fma(dGg[bp], acc, (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]));
             if (bp == gi - 2) accg[i] = acc;
         }
-        __threadfence();
         __syncthreads();
         if (tid == 0) ht_post(&ctr[1 + h], tag | (unsigned long long)(bp + 1));
     }
@@ -216,10 +221,16 @@ extern "C" __global__ void __launch_bounds__(HT_NT) k_ht_lead(double *A, double 
             double rc = rcv[w - 1], rs = rsv[w - 1], b00 = sB[(w - 2) * (HT_R + 1)];
             double kcc = 0.0, kss = 0.0, kr2 = 0.0;                 // lane k: G and B(k, k) of position k
             for (int q = w - 1; q >= 1; --q) {
-                double xq = 0.0, yq = 0.0, rcn = 0.0, rsn = 0.0, b00n = 0.0;
+                double xq = 0.0, yq = 0.0, rcn = 0.0, rsn = 0.0, b00n = 0.0, pfn = pf;
+                // the next position's operand of this lane: row q-2 (k >= q) or column q-2 (k <= q-3),
+                // loaded first so that its latency hides behind this position's chain.  No store of
+                // this position touches these entries except B(q-2, q) (lane q's), which lane q-2
+                // finalises below and lane q then takes from its register.
+                const bool pr = k >= q && k < w, pl = q >= 2 && (pr || k <= q - 3);
                 if (q >= 2) {
                     xq = rdlane(cv, q - 2); yq = rdlane(pf, q - 2);
                     rcn = rcv[q - 1]; rsn = rsv[q - 1]; b00n = sB[(q - 2) * (HT_R + 1)];
+                    if (pl && k != q) pfn = sB[pr ? (q - 2) + HT_R * k : k + HT_R * (q - 2)];
                 }
                 const double nb00 = rc * b00, nb10 = -rs * b00;
                 const double nb01 = rc * b01 + rs * b11, nb11 = rc * b11 - rs * b01;
@@ -237,8 +248,11 @@ extern "C" __global__ void __launch_bounds__(HT_NT) k_ht_lead(double *A, double 
                 if (rl || gl) sB[rl ? q + HT_R * k : k + HT_R * q] = rl ? v : u;   // row q / column q final
                 cv = rl ? u : (gl ? v : (k == q ? n01 : cv));      // carried: row q-1 / column q-1
                 if (k == q) { kcc = cc; kss = ss; kr2 = r2; }
-                const bool pr = k >= q && k < w;
-                if (q >= 2 && (pr || k <= q - 3)) pf = sB[pr ? (q - 2) + HT_R * k : k + HT_R * (q - 2)];
+                if (q >= 2) {
+                    const double u2 = rdlane(u, q - 2);             // B(q-2, q) after G_c (lane q-2: column lane)
+                    if (k == q && pl) pfn = u2;
+                }
+                pf = pfn;
                 rc = rcn; rs = rsn; b00 = b00n;
                 wave_sync();
             }
