@@ -349,18 +349,19 @@ def test_hgeqz_paths_with_aed_and_multishift():
     assert ratio < 1.0
 
 
-@pytest.mark.parametrize("chain", [0, 4, 20])
-def test_overlapped_sweep_matches_serial_order(chain):
+@pytest.mark.parametrize("chain,nw", [(0, 0), (4, 0), (20, 0), (0, 200)])
+def test_overlapped_sweep_matches_serial_order(chain, nw):
     """n=1500 with the overlapped sweep updates forced on for every active block (gqz_overlap_min):
     the far left and the right updates of a round on the second stream while the next round is
     chased, and each chain's near-column update applied by its next chase launch.  Chains of 10
-    shifts (and a 4-shift remainder), 4 and 20 shifts give different window steps and near widths.
-    The serial order on the same pencil is the comparison: both meet the residual bounds on the
-    host and match LAPACK's eigenvalues condition-weighted."""
+    shifts (and a 4-shift remainder), 4 and 20 shifts give different window steps and near widths;
+    AED windows of 200 add the nested QZs of the windows to the overlapped sweeps.  The serial
+    order on the same pencil is the comparison: both meet the residual bounds on the host and match
+    LAPACK's eigenvalues condition-weighted."""
     torch, gqz, O = _mods()
     L = gqz.lib()
     n = 1500
-    _note("overlapped sweep, chain", chain, n)
+    _note("overlapped sweep, chain", chain, "nw", nw, n)
     H, T = QC.pencil_ht_random(n, 17)
     prev_on, prev_min = L.gqz_overlap(-1), L.gqz_overlap_min(-1)
     try:
@@ -371,6 +372,7 @@ def test_overlapped_sweep_matches_serial_order(chain):
             Qd = gqz.colmajor(n); Zd = gqz.colmajor(n)
             prm = gqz.GqzParams()
             prm.chain = chain
+            prm.nw = nw
             ar, ai, be, info, st = gqz.pdhgeqz("S", "I", "I", Hd, Td, Qd, Zd, params=prm)
             print("overlap", ovl, "info", info, "sweeps", st["sweeps"], "windows", st["chase_windows"], flush=True)
             assert info == 0 and st["sweeps"] > 0
