@@ -90,14 +90,13 @@ __device__ __forceinline__ void d_larfg2(double &alpha, double &x1, double &tau)
 // rounding of tau itself stays unbiased (round to nearest in all but rare cases), since a biased
 // tau makes long products of reflectors lose orthogonality coherently.
 
-// 1/x
-__device__ __forceinline__ double d_rcp(double x) {
-    const double ax = fabs(x);
-    if (!(ax > 1e-290 && ax < 1e290)) return 1.0 / x;
+// 1/x for 1e-290 < |x| < 1e290, without the range test: the callers below have bounded their
+// operands already (|alpha - beta| and d within a small factor of an operand checked to lie in
+// (1e-140, 1e140), |p0| at least sqrt(nw) of a checked nw, every tau denominator in [1, 3])
+__device__ __forceinline__ double d_rcpn(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(r, fma(-x, r, 1.0), r);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return r;
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 // sqrt(x) for 1e-290 < x < 1e290 (no scaling needed): g ~ sqrt(x), h ~ 1 / (2 sqrt(x)) from the
@@ -118,9 +117,9 @@ __device__ __forceinline__ void d_larfg3f(double &alpha, double &x1, double &x2,
     const double s = fma(x1, x1, x2 * x2);
     if (s == 0.0) { tau = 0.0; return; }
     const double beta = -copysign(d_sqrtf(fma(alpha, alpha, s)), alpha);
-    const double rd = d_rcp(alpha - beta);
+    const double rd = d_rcpn(alpha - beta);
     x1 *= rd; x2 *= rd;
-    tau = 2.0 * d_rcp(fma(x1, x1, fma(x2, x2, 1.0)));
+    tau = 2.0 * d_rcpn(fma(x1, x1, fma(x2, x2, 1.0)));
     alpha = beta;
 }
 
@@ -131,8 +130,8 @@ __device__ __forceinline__ void d_larfg2f(double &alpha, double &x1, double &tau
     const double m = fmax(fabs(alpha), fabs(x1));
     if (!(m > 1e-140 && m < 1e140)) { d_larfg2(alpha, x1, tau); return; }
     const double beta = -copysign(d_sqrtf(fma(alpha, alpha, x1 * x1)), alpha);
-    x1 *= d_rcp(alpha - beta);
-    tau = 2.0 * d_rcp(fma(x1, x1, 1.0));
+    x1 *= d_rcpn(alpha - beta);
+    tau = 2.0 * d_rcpn(fma(x1, x1, 1.0));
     alpha = beta;
 }
 
@@ -143,7 +142,7 @@ __device__ __forceinline__ void d_lartgf(double f, double g, double &c, double &
     const double m = fmax(fabs(f), fabs(g));
     if (!(m > 1e-140 && m < 1e140)) { d_lartg(f, g, c, s, r); return; }
     const double d = d_sqrtf(fma(f, f, g * g));
-    const double rd = d_rcp(d);
+    const double rd = d_rcpn(d);
     c = fabs(f) * rd;
     s = (f > 0.0 ? g : -g) * rd;
     r = copysign(d, f);
@@ -215,9 +214,9 @@ __device__ __forceinline__ void d_invhse3f(double r10, double r11, double r12, d
     w0 -= s1 * x0; w1 -= s1 * x1;
     const double w2 = -s1;
     const double p0 = w0 + (w0 >= 0.0 ? 1.0 : -1.0);
-    const double rp = d_rcp(p0);
+    const double rp = d_rcpn(p0);
     const double p1 = w1 * rp, p2 = w2 * rp;
-    tau = 2.0 * d_rcp(fma(p1, p1, fma(p2, p2, 1.0)));
+    tau = 2.0 * d_rcpn(fma(p1, p1, fma(p2, p2, 1.0)));
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
@@ -238,9 +237,9 @@ __device__ __forceinline__ void d_nullrefl3(double r10, double r11, double r12, 
         return;
     }
     const double p0 = w0 + copysign(d_sqrtf(nw), w0);
-    const double rp = d_rcp(p0);
+    const double rp = d_rcpn(p0);
     const double p1 = w1 * rp, p2 = w2 * rp;
-    tau = 2.0 * d_rcp(fma(p1, p1, fma(p2, p2, 1.0)));
+    tau = 2.0 * d_rcpn(fma(p1, p1, fma(p2, p2, 1.0)));
     v[0] = 1.0; v[1] = p1; v[2] = p2;
 }
 
