@@ -492,6 +492,87 @@ hipEvent_t sync_event(size_t i) {
 // blocks below L.overlap_min (the nested QZ of AED windows, the end of a top-level QZ) keep the updates
 // on the chase stream: their slab updates are short and the stream ordering would cost more than it hides
 
+// One round of the chain scheduler (host only, no device state).  A chain launched in a round gets a
+// record: its window [p, p + w), the chase steps, whether it is the chain's last window, nc (the near
+// columns its U leaves pending for its next launch, overlap only) and near/pp/pw (the pending block
+// of its previous window that this launch applies first).
+struct SchedWin { int chain, p, w, steps, last, intro, nc, near, pp, pw; };
+
+int sched_select(const std::vector<Chain> &ch, int ktop, int kbot, int n, bool ovl, int wmax, int nslot,
+                 SchedWin *out) {
+    const int nch = (int)ch.size();
+    int nw = 0;
+    // rows from which the chains ahead forbid access; nlim: columns from which the windows of the
+    // chains ahead start, this round and later (a pending near block must end above it)
+    int limit = kbot + 1, nlim = kbot + 1;
+    int active = 0;
+    for (int c = 0; c < nch && nw < nslot; ++c) {
+        const Chain &x = ch[c];
+        const int pp = x.prevp;
+        if (x.done) {
+            if (ovl) limit = std::min(limit, pp);
+            continue;
+        }
+        if (!x.started && active >= nslot) break;
+        const int p = x.started ? x.gtop : ktop;
+        const int w = std::min(wmax, kbot - p + 1);
+        const int last = (p + w - 1 == kbot);
+        const int steps = last ? w : (w - 3 * x.nb - 1);
+        int nc = 0;
+        if (!last) {
+            const int np = p + steps, nw2 = std::min(wmax, kbot - np + 1);
+            nc = std::max(0, std::min(np + nw2, n) - (p + w));
+        }
+        if (p + w > limit || (ovl && nc > 0 && p + w + nc > nlim)) {
+            // wait: the chain ahead still occupies rows below; nothing behind may pass
+            limit = std::min(limit, x.started ? x.gtop : ktop);
+            if (ovl) limit = std::min(limit, pp);
+            nlim = std::min(nlim, p);
+            if (!x.started) break;
+            ++active;
+            continue;
+        }
+        SchedWin &o = out[nw++];
+        o.chain = c; o.p = p; o.w = w; o.steps = steps; o.last = last; o.intro = x.started ? 0 : 1;
+        o.nc = ovl ? nc : 0;
+        o.near = ovl ? x.pnc : 0; o.pp = x.prevp; o.pw = x.pw;
+        limit = p;
+        if (ovl) limit = std::min(limit, pp);
+        nlim = std::min(nlim, p);
+        ++active;
+    }
+    return nw;
+}
+
+// after a launch: a finished chain's window start constrains the next round only; the launched
+// chains move on
+void sched_commit(std::vector<Chain> &ch, const SchedWin *sw, int nw) {
+    for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
+    for (int q = 0; q < nw; ++q) {
+        Chain &x = ch[sw[q].chain];
+        x.started = true;
+        x.gtop = sw[q].p + sw[q].steps;
+        x.prevp = sw[q].p;
+        x.pw = sw[q].w;
+        x.pnc = sw[q].nc;
+        ++x.nl;
+        if (sw[q].last) x.done = true;
+    }
+}
+
+// no window could move: with overlap the only possible obstacle is the last right update of a finished
+// chain; once stream 2 has finished, those constraints go.  Returns false when there is none (a stall).
+bool sched_unblock(std::vector<Chain> &ch) {
+    bool pend = false;
+    for (auto &x : ch) pend = pend || (x.done && x.prevp != INT32_MAX);
+    for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
+    return pend;
+}
+
+void sched_reset(std::vector<Chain> &ch) {
+    for (auto &x : ch) { x.prevp = INT32_MAX; x.nl = 0; x.pw = 0; x.pnc = 0; }
+}
+
 void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> &ch, const double *d_sr,
                   const double *d_si, const double *d_sb) {
     const int WMAX = gqz_chase_max_window();
@@ -503,7 +584,7 @@ void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> 
     // without overlap: slots by launch position and round parity; with overlap: two per chain
     const size_t nslots = ovl ? 2 * (size_t)nch : 2 * (size_t)nslot;
     double *U = L.ws.alloc(nslots * slot), *V = L.ws.alloc(nslots * slot);
-    for (auto &x : ch) { x.prevp = INT32_MAX; x.nl = 0; x.pw = 0; x.pnc = 0; }
+    sched_reset(ch);
     int first = 0;                       // oldest chain not yet done
     long round = 0;
     bool b_pend[2] = { false, false };   // stream 2 holds updates recorded in B[parity]
@@ -542,70 +623,33 @@ void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> 
     };
     while (first < nch) {
         const int par = (int)(round & 1);
-        GqzChainSet S = {};
-        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = U; S.V = V; S.ustride = WMAX * WMAX;
-        S.sr = d_sr; S.si = d_si; S.sb = d_sb; S.smlnum = v.smlnum;
-        Win ws[GQZ_MAXCH];
-        int slot_chain[GQZ_MAXCH], nearc[GQZ_MAXCH];
-        int nw = 0;
-        // rows from which the chains ahead forbid access; nlim: columns from which the windows of
-        // the chains ahead start, this round and later (a pending near block must end above it)
-        int limit = kbot + 1, nlim = kbot + 1;
-        int active = 0;
-        for (int c = 0; c < nch && nw < nslot; ++c) {
-            Chain &x = ch[c];
-            const int pp = x.prevp;
-            if (x.done) {
-                if (ovl) limit = std::min(limit, pp);
-                continue;
-            }
-            if (!x.started && active >= nslot) break;
-            int p = x.started ? x.gtop : ktop;
-            int w = std::min(WMAX, kbot - p + 1);
-            const int last = (p + w - 1 == kbot);
-            const int steps = last ? w : (w - 3 * x.nb - 1);
-            int nc = 0;
-            if (!last) {
-                const int np = p + steps, nw2 = std::min(WMAX, kbot - np + 1);
-                nc = std::max(0, std::min(np + nw2, v.n) - (p + w));
-            }
-            if (p + w > limit || (ovl && nc > 0 && p + w + nc > nlim)) {
-                // wait: the chain ahead still occupies rows below; nothing behind may pass
-                limit = std::min(limit, x.started ? x.gtop : ktop);
-                if (ovl) limit = std::min(limit, pp);
-                nlim = std::min(nlim, p);
-                if (!x.started) break;
-                ++active;
-                continue;
-            }
-            const int q = nw++;
-            S.p[q] = p; S.w[q] = w; S.nb[q] = x.nb; S.soff[q] = x.soff;
-            S.intro[q] = x.started ? 0 : 1; S.t0[q] = x.started ? 0 : -1; S.last[q] = last;
-            S.steps[q] = steps;
-            S.uslot[q] = ovl ? 2 * c + (x.nl & 1) : par * nslot + q;
-            if (ovl && x.pnc > 0) {
-                S.near[q] = x.pnc; S.pslot[q] = 2 * c + ((x.nl - 1) & 1); S.pp[q] = x.prevp; S.pw[q] = x.pw;
-            }
-            nearc[q] = ovl ? nc : 0;
-            ws[q] = Win{ p, w, U + (size_t)S.uslot[q] * slot, V + (size_t)S.uslot[q] * slot };
-            slot_chain[q] = c;
-            limit = p;
-            if (ovl) limit = std::min(limit, pp);
-            nlim = std::min(nlim, p);
-            ++active;
-        }
+        SchedWin sw[GQZ_MAXCH];
+        const int nw = sched_select(ch, ktop, kbot, v.n, ovl, WMAX, nslot, sw);
         if (nw == 0) {
-            bool pend = false;
-            for (auto &x : ch) pend = pend || (x.done && x.prevp != INT32_MAX);
-            if (!ovl || !pend) throw GqzError("chain scheduler made no progress");
-            // the only obstacle is the last right update of a finished chain: let stream 2 finish
+            if (!ovl || !sched_unblock(ch)) throw GqzError("chain scheduler made no progress");
+            // let stream 2 finish (the finished chain's right update was the obstacle)
             launch_b();
             wait_b(0); wait_b(1);
             b_pend[0] = b_pend[1] = false;
-            for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
             continue;
         }
+        GqzChainSet S = {};
+        S.H = v.H; S.T = v.T; S.ld = v.ld; S.U = U; S.V = V; S.ustride = WMAX * WMAX;
+        S.sr = d_sr; S.si = d_si; S.sb = d_sb; S.smlnum = v.smlnum;
         S.nch = nw;
+        Win ws[GQZ_MAXCH];
+        for (int q = 0; q < nw; ++q) {
+            const SchedWin &o = sw[q];
+            const Chain &x = ch[o.chain];
+            S.p[q] = o.p; S.w[q] = o.w; S.nb[q] = x.nb; S.soff[q] = x.soff;
+            S.intro[q] = o.intro; S.t0[q] = o.intro ? -1 : 0; S.last[q] = o.last;
+            S.steps[q] = o.steps;
+            S.uslot[q] = ovl ? 2 * o.chain + (x.nl & 1) : par * nslot + q;
+            if (o.near > 0) {
+                S.near[q] = o.near; S.pslot[q] = 2 * o.chain + ((x.nl - 1) & 1); S.pp[q] = o.pp; S.pw[q] = o.pw;
+            }
+            ws[q] = Win{ o.p, o.w, U + (size_t)S.uslot[q] * slot, V + (size_t)S.uslot[q] * slot };
+        }
         if (ovl) {
             wait_b(par);                 // B(r-2)
             launch_b();                  // B(r-1), released together with this chase
@@ -616,25 +660,14 @@ void chase_chains(Run &R, const View &v, int ktop, int kbot, std::vector<Chain> 
         gqz_chase_set(L.st, S);
         kt_end(kc, 0.0, by);
         R.st.chase_windows += nw;
-        // a finished chain's window start constrains the next round only
-        for (auto &x : ch) if (x.done) x.prevp = INT32_MAX;
         if (!ovl) {
             apply_windows(R, v, ws, nw);
         } else {
-            for (int q = 0; q < nw; ++q) { pws[q] = ws[q]; pnear[q] = nearc[q]; }
+            for (int q = 0; q < nw; ++q) { pws[q] = ws[q]; pnear[q] = sw[q].nc; }
             pnw = nw;
             ppar = par;
         }
-        for (int q = 0; q < nw; ++q) {
-            Chain &x = ch[slot_chain[q]];
-            x.started = true;
-            x.gtop = S.p[q] + S.steps[q];
-            x.prevp = S.p[q];
-            x.pw = S.w[q];
-            x.pnc = nearc[q];
-            ++x.nl;
-            if (S.last[q]) x.done = true;
-        }
+        sched_commit(ch, sw, nw);
         while (first < nch && ch[first].done) ++first;
         ++round;
     }
@@ -1134,6 +1167,41 @@ int gqz_overlap_min(int n) {
     const int prev = L.overlap_min;
     if (n >= 0) L.overlap_min = n;
     return prev;
+}
+int gqz_sched_replay(int n, int ktop, int kbot, int nch, const int *nb, int ovl, int *out, int cap) {
+    const int wmax = gqz_chase_max_window();
+    if (n < 1 || ktop < 0 || kbot >= n || kbot - ktop < 2 || nch < 1 || !nb || cap < 0 || (cap > 0 && !out))
+        return -1;
+    std::vector<Chain> ch(nch);
+    for (int c = 0; c < nch; ++c) {
+        if (nb[c] < 1 || 3 * nb[c] + 2 > wmax) return -1;
+        ch[c] = Chain{ 0, nb[c], ktop, false, false, INT32_MAX, 0, 0, 0 };
+    }
+    sched_reset(ch);
+    int first = 0, rec = 0, round = 0;
+    auto put = [&](const int *r) {
+        if (rec < cap) for (int k = 0; k < 8; ++k) out[8 * rec + k] = r[k];
+        ++rec;
+    };
+    while (first < nch) {
+        SchedWin sw[GQZ_MAXCH];
+        const int nw = sched_select(ch, ktop, kbot, n, ovl != 0, wmax, GQZ_MAXCH, sw);
+        if (nw == 0) {
+            if (!ovl || !sched_unblock(ch)) return -1;
+            const int r[8] = { round, -1, 0, 0, 0, 0, 0, 0 };       // stream 2 drained
+            put(r);
+            continue;
+        }
+        for (int q = 0; q < nw; ++q) {
+            const SchedWin &o = sw[q];
+            const int r[8] = { round, o.chain, o.p, o.w, o.steps, o.last, o.nc, o.near };
+            put(r);
+        }
+        sched_commit(ch, sw, nw);
+        while (first < nch && ch[first].done) ++first;
+        if (++round > 100000000) return -1;
+    }
+    return rec;
 }
 int gqz_trace_get(gqz_trace_rec *out, int cap) {
     int k = std::min<int>(cap, (int)L.trace.size());

@@ -44,7 +44,7 @@ POLICY_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINT
                         C.POINTER(C.c_int), C.POINTER(C.c_int))
 
 EXPORTS = ["gqz_init", "gqz_finalize", "gqz_version", "gqz_stream", "gqz_sync", "gqz_set_policy",
-           "gqz_trace_enable", "gqz_trace_get", "gqz_progress", "gqz_overlap", "gqz_overlap_min", "gqz_last_error", "gqz_hgeqz_dev", "gqz_hgeqz",
+           "gqz_trace_enable", "gqz_trace_get", "gqz_progress", "gqz_overlap", "gqz_overlap_min", "gqz_sched_replay", "gqz_last_error", "gqz_hgeqz_dev", "gqz_hgeqz",
            "gqz_gghrd_dev", "gqz_gghrd", "gqz_geqrb_dev", "gqz_pdhgeqz_host", "gqz_pdgghrd_host",
            "gqz_compres_dev", "gqz_batched_dev", "gqz_mlp_forward_dev", "gqz_mlp_train_step_dev"]
 
@@ -83,6 +83,7 @@ def lib():
     L.gqz_progress.argtypes = [C.c_double]
     L.gqz_overlap.argtypes = [C.c_int]
     L.gqz_overlap_min.argtypes = [C.c_int]
+    L.gqz_sched_replay.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int]
     L.gqz_slab_update_dev.argtypes = [C.c_char, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int]
     L.gqz_chase_window_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.c_int, vp, vp, vp, vp, vp]

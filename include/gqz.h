@@ -71,6 +71,14 @@ int gqz_overlap(int on);
  * the nested QZs of AED windows, keep their updates on the library stream).  n < 0 only queries.
  * Returns the previous value; per host thread.  No counterpart in the reference. */
 int gqz_overlap_min(int n);
+/* Replay of the sweep's chain scheduler on the host, without a device (a test hook; no counterpart in the
+ * reference): nch chains of nb[c] bulges (1 <= nb[c], 3 nb[c] + 2 <= 64) chased over rows [ktop, kbot]
+ * of an order-n pencil, with the overlapped updates (ovl = 1) or without.  Writes one record of 8 ints
+ * per window launched: round, chain, p, w, steps, last, nc (near columns its U leaves pending), near
+ * (the pending columns of its previous window that this launch applies first); a record with chain = -1
+ * marks a point where the scheduler let the second stream finish.  Returns the number of records
+ * (only the first cap are written) or -1 (bad arguments, or the scheduler stalled). */
+int gqz_sched_replay(int n, int ktop, int kbot, int nch, const int *nb, int ovl, int *out, int cap);
 const char *gqz_last_error(void);
 
 /* ---- QZ ----------------------------------------------------------------------------------
